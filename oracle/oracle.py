@@ -1,0 +1,435 @@
+"""FP64 CPU oracle for the reciprocal-space S(q) path (Python side).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` import this module, and only as the checker
+or the timed CPU restatement; the product (``sqcalc_amd``) never does.
+
+It wraps ``libsqc_oracle.so`` (``sqc_oracle.c``, ``sqc_nufft_cpu.c``) and
+restates the host-side bookkeeping of the reference in numpy:
+
+* ``shell_table``   S(q) and partial columns, src/sqc_structure_factor.f90:657-719
+* ``format_table``  the ``# q S(q) S(a-b)...`` text, src/sqc_structure_factor.f90:771-827
+* ``denominators``  mode_denominator summed per shell, :389-451
+* ``phihat``        Fourier coefficients of the ES kernel by Gauss-Legendre
+                    quadrature (the role of external/finufft/src/cuda/common.cu:27-66)
+
+How each function is pinned is listed in the header of ``sqc_oracle.c`` and in
+tests/test_oracle.py.  ``nufft_frame`` (the CPU restatement of the reference
+FINUFFT path) is pinned by the direct sum.
+"""
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# IT92 X-ray coefficients (a1..a4, b1..b4, c) and bound coherent neutron
+# lengths b [fm] of the species the tests and benchmarks use.  Values are the
+# published tables the reference carries (src/sqc_element_data.f90; IT92
+# 6.1.1.4, Neutron News 3 (1992) 29-37).
+NEUTRON_B = {"H": -3.739, "C": 6.646, "O": 5.803, "Na": 3.63, "Al": 3.449, "Si": 4.1491,
+             "Cl": 9.577, "Ca": 4.7, "Ni": 10.3}
+XRAY = {
+    "H": ([0.493002, 0.322912, 0.140191, 0.04081], [10.5109, 26.1257, 3.14236, 57.7997], 0.003038),
+    "C": ([2.31, 1.02, 1.5886, 0.865], [20.8439, 10.2075, 0.5687, 51.6512], 0.2156),
+    "O": ([3.0485, 2.2868, 1.5463, 0.867], [13.2771, 5.7011, 0.3239, 32.9089], 0.2508),
+    "Na": ([4.7626, 3.1736, 1.2674, 1.1128], [3.285, 8.8422, 0.3136, 129.424], 0.676),
+    "Al": ([6.4202, 1.9002, 1.5936, 1.9646], [3.0387, 0.7426, 31.5472, 85.0886], 1.1151),
+    "Si": ([6.2915, 3.0353, 1.9891, 1.541], [2.4386, 32.3337, 0.6785, 81.6937], 1.1407),
+    "Cl": ([11.4604, 7.1964, 6.2556, 1.6455], [0.0104, 1.1662, 18.5194, 47.7784], -9.5574),
+    "Ca": ([8.6266, 7.3873, 1.5899, 1.0211], [10.4421, 0.6599, 85.7484, 178.437], 1.3751),
+    "Ni": ([12.8376, 7.292, 4.4438, 2.38], [3.8785, 0.2565, 12.1763, 66.3421], 1.0341),
+    "O2-": ([3.7504, 2.84294, 1.54298, 1.62091], [16.5151, 6.59203, 0.319201, 43.3486], 0.24206),
+    "Si4+": ([4.43918, 3.20345, 1.19453, 0.41653], [1.64167, 3.43757, 0.2149, 6.65365], 0.746297),
+}
+WEIGHT_KIND = {"unit": 0, "neutron": 1, "xray": 2}
+NORM_KIND = {"mean": 1, "self": 2, "n": 3}
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+
+
+def lib():
+    """Load (building on first use when the compiler is at hand) the oracle."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(HERE, "libsqc_oracle.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    L = ctypes.CDLL(path)
+    L.sqo_cell_setup.restype = ctypes.c_double
+    L.sqo_cell_setup.argtypes = [_d, _d]
+    L.sqo_cell_from_bounds.argtypes = [_d, _d, _d, _d, _d]
+    L.sqo_mode_extent.argtypes = [_d, ctypes.c_double, _i64]
+    L.sqo_mode_count.restype = ctypes.c_int64
+    L.sqo_mode_count.argtypes = [_d, ctypes.c_double, ctypes.c_double]
+    L.sqo_modes.restype = ctypes.c_int64
+    L.sqo_modes.argtypes = [_d, ctypes.c_double, ctypes.c_double, ctypes.c_int, _i32, _d, _d,
+                            _i64, _i32]
+    L.sqo_denominators.argtypes = [ctypes.c_int64, _d, _i32, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, _i64, _d, _d, ctypes.c_int64,
+                                   _d, _i64]
+    L.sqo_direct_rho.restype = ctypes.c_int
+    L.sqo_direct_rho.argtypes = [ctypes.c_int64, _d, _d, _d, _i32, ctypes.c_int, ctypes.c_int64,
+                                 _i32, _d, ctypes.c_int, _d]
+    L.sqo_shell_sums.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _i32, ctypes.c_int,
+                                 ctypes.c_int, _d, _d, _d, _d]
+    L.sqo_direct_grad.restype = ctypes.c_int
+    L.sqo_direct_grad.argtypes = [ctypes.c_int64, _d, _d, _d, _i32, ctypes.c_int, ctypes.c_int64,
+                                  _d, _d, _i32, _d, ctypes.c_int, _d, _d, _d, _d, _d, _i64,
+                                  ctypes.c_int, _d]
+    L.sqo_nufft_spread.restype = ctypes.c_int
+    L.sqo_nufft_spread.argtypes = [ctypes.c_int64, _d, _d, _d, _i32, _d, ctypes.c_int, _i64,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_int64, _d]
+    L.sqo_nufft_bin.argtypes = [ctypes.c_int, _d, _i64, ctypes.c_int64, _i32, _d, _i32,
+                                ctypes.c_int, _d, _d, _d, ctypes.c_int, _d, _d, _d, _d]
+    L.sqo_num_threads.restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# geometry and mode set
+# --------------------------------------------------------------------------
+
+def cell_from_bounds(lo, hi, tilt=(0.0, 0.0, 0.0)):
+    """(A, origin): A holds a1, a2, a3 as columns (src/sqc_cell.f90:39-63)."""
+    a = np.zeros(9)
+    o = np.zeros(3)
+    lib().sqo_cell_from_bounds(_p(_f64(lo), _d), _p(_f64(hi), _d), _p(_f64(tilt), _d),
+                               _p(a, _d), _p(o, _d))
+    return a.reshape(3, 3, order="F"), o
+
+
+def reciprocal(cell):
+    """B with b1, b2, b3 as columns, B = 2 pi A^-T (src/sqc_cell.f90:76-86)."""
+    a = _f64(np.asarray(cell).reshape(3, 3).ravel(order="F"))
+    b = np.zeros(9)
+    lib().sqo_cell_setup(_p(a, _d), _p(b, _d))
+    return b.reshape(3, 3, order="F")
+
+
+def modes(cell, qmin, qmax, nq):
+    """Kept reciprocal-lattice modes of sf_configure (src/sqc_structure_factor.f90:327-399)."""
+    a = _f64(np.asarray(cell).reshape(3, 3).ravel(order="F"))
+    L = lib()
+    m = np.zeros(3, dtype=np.int64)
+    L.sqo_mode_extent(_p(a, _d), qmax, _p(m, _i64))
+    k = int(L.sqo_mode_count(_p(a, _d), qmin, qmax))
+    hkl = np.zeros((k, 3), dtype=np.int32)
+    qvec = np.zeros((k, 3))
+    qlen = np.zeros(k)
+    gidx = np.zeros(k, dtype=np.int64)
+    shell = np.zeros(k, dtype=np.int32)
+    L.sqo_modes(_p(a, _d), qmin, qmax, nq, _p(hkl, _i32), _p(qvec, _d), _p(qlen, _d),
+                _p(gidx, _i64), _p(shell, _i32))
+    return {"m": m, "hkl": hkl, "qvec": qvec, "qlen": qlen, "gidx": gidx, "shell": shell,
+            "qmin": float(qmin), "qmax": float(qmax), "nq": int(nq)}
+
+
+def weight_params(kind, labels):
+    """Per-species constants (unit/neutron) or IT92 coefficients (xray)."""
+    S = len(labels)
+    amp = np.ones(S)
+    xcoef = np.zeros((S, 9))
+    for i, lab in enumerate(labels):
+        if kind == "neutron":
+            elem = lab.rstrip("+-0123456789").replace("val", "")
+            amp[i] = NEUTRON_B[elem]
+        elif kind == "xray":
+            a, b, c = XRAY[lab]
+            xcoef[i] = list(a) + list(b) + [c]
+    return amp, xcoef
+
+
+def amplitude(kind, amp, xcoef, sp, q):
+    q = np.asarray(q, dtype=np.float64)
+    if kind == "unit":
+        return np.ones_like(q)
+    if kind == "neutron":
+        return np.full_like(q, amp[sp])
+    c = xcoef[sp]
+    st = (q / (4.0 * math.acos(-1.0))) ** 2
+    f = np.full_like(q, c[8])
+    for i in range(4):
+        f = f + c[i] * np.exp(-c[4 + i] * st)
+    return f
+
+
+def denominators(md, norm, kind, counts, amp, xcoef, natoms):
+    """den[nq], shell_count[nq] (src/sqc_structure_factor.f90:389-451)."""
+    nq = md["nq"]
+    den = np.zeros(nq)
+    cnt = np.zeros(nq, dtype=np.int64)
+    counts = np.ascontiguousarray(counts, dtype=np.int64)
+    lib().sqo_denominators(len(md["qlen"]), _p(md["qlen"], _d), _p(md["shell"], _i32), nq,
+                           NORM_KIND[norm], WEIGHT_KIND[kind], len(counts), _p(counts, _i64),
+                           _p(_f64(amp), _d), _p(_f64(xcoef), _d), natoms, _p(den, _d),
+                           _p(cnt, _i64))
+    return den, cnt
+
+
+# --------------------------------------------------------------------------
+# direct sum
+# --------------------------------------------------------------------------
+
+def direct_rho(pos, cell, origin, species_of, nspecies, md, method=1):
+    """rho_a(q_m), complex [S, K] (src/sqc_structure_factor.f90:1398-1475)."""
+    a = _f64(np.asarray(cell).reshape(3, 3).ravel(order="F"))
+    pos = _f64(pos)
+    sp = np.ascontiguousarray(species_of, dtype=np.int32)
+    K = len(md["qlen"])
+    rho = np.zeros((nspecies, K, 2))
+    rc = lib().sqo_direct_rho(len(sp), _p(pos, _d), _p(a, _d), _p(_f64(origin), _d),
+                              _p(sp, _i32), nspecies, K, _p(md["hkl"], _i32),
+                              _p(_f64(md["qvec"]), _d), method, _p(rho, _d))
+    if rc:
+        raise MemoryError("oracle direct sum")
+    return rho[..., 0] + 1j * rho[..., 1]
+
+
+def shell_sums(rho, md, kind, amp, xcoef, partials=True):
+    """num[nq] and partial[S, S, nq] of one frame, in mode order."""
+    S, K = rho.shape
+    nq = md["nq"]
+    r = np.ascontiguousarray(np.stack([rho.real, rho.imag], axis=-1))
+    num = np.zeros(nq)
+    part = np.zeros((S, S, nq)) if partials else None
+    lib().sqo_shell_sums(S, K, _p(r, _d), _p(md["qlen"], _d), _p(md["shell"], _i32), nq,
+                         WEIGHT_KIND[kind], _p(_f64(amp), _d), _p(_f64(xcoef), _d),
+                         _p(num, _d), _p(part, _d) if partials else None)
+    return num, part
+
+
+def gradient(pos, cell, origin, species_of, md, rho, kind, amp, xcoef, gS, den, shell_count,
+             gP=None):
+    """Analytic FP64 dL/dr (3 per atom) for upstream gradients on the shell values."""
+    a = _f64(np.asarray(cell).reshape(3, 3).ravel(order="F"))
+    pos = _f64(pos)
+    sp = np.ascontiguousarray(species_of, dtype=np.int32)
+    S, K = rho.shape
+    r = np.ascontiguousarray(np.stack([rho.real, rho.imag], axis=-1))
+    grad = np.zeros((len(sp), 3))
+    gP_arr = _f64(gP) if gP is not None else None
+    rc = lib().sqo_direct_grad(len(sp), _p(pos, _d), _p(a, _d), _p(_f64(origin), _d),
+                               _p(sp, _i32), S, K, _p(_f64(md["qvec"]), _d),
+                               _p(md["qlen"], _d), _p(md["shell"], _i32), _p(r, _d),
+                               WEIGHT_KIND[kind], _p(_f64(amp), _d), _p(_f64(xcoef), _d),
+                               _p(_f64(gS), _d), _p(_f64(den), _d),
+                               _p(gP_arr, _d) if gP is not None else None,
+                               _p(np.ascontiguousarray(shell_count, dtype=np.int64), _i64),
+                               md["nq"], _p(grad, _d))
+    if rc:
+        raise MemoryError("oracle gradient")
+    return grad
+
+
+# --------------------------------------------------------------------------
+# whole-run helpers (shell table, text format)
+# --------------------------------------------------------------------------
+
+class Run:
+    """Accumulates frames like structure_factor_t (static subset).
+
+    types: LAMMPS type id per atom; labels: {type_id: species label} or None.
+    Species are the present type ids in ascending order
+    (src/sqc_structure_factor.f90:1109-1163).
+    """
+
+    def __init__(self, cell, origin, types, labels=None, weight="unit", norm="mean",
+                 qmin=0.0, qmax=20.0, nq=500, partials=True, faber_ziman=False):
+        self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+        self.origin = np.asarray(origin, dtype=np.float64)
+        types = np.asarray(types, dtype=np.int64)
+        self.types = types
+        self.ntypes = max(int(types.max()), max(labels) if labels else 1)
+        self.species_type = np.unique(types)
+        self.species_of = np.searchsorted(self.species_type, types).astype(np.int32)
+        self.counts = np.array([np.count_nonzero(types == t) for t in self.species_type])
+        self.labels = labels or {}
+        self.weight = weight
+        self.norm = norm
+        self.partials = partials
+        self.faber_ziman = faber_ziman
+        names = [self.labels.get(int(t), str(int(t))) for t in self.species_type]
+        self.amp, self.xcoef = (weight_params(weight, names) if weight != "unit"
+                                else (np.ones(len(names)), np.zeros((len(names), 9))))
+        self.md = modes(self.cell, qmin, qmax, nq)
+        self.natoms = len(types)
+        self.den, self.shell_count = denominators(self.md, norm, weight, self.counts, self.amp,
+                                                  self.xcoef, self.natoms)
+        S = len(self.species_type)
+        self.num = np.zeros(nq)
+        self.partial = np.zeros((S, S, nq))
+        self.nframes = 0
+
+    def add_frame(self, pos, method=1):
+        rho = direct_rho(pos, self.cell, self.origin, self.species_of, len(self.species_type),
+                         self.md, method=method)
+        num, part = shell_sums(rho, self.md, self.weight, self.amp, self.xcoef, True)
+        self.num += num
+        self.partial += part
+        self.nframes += 1
+        return rho
+
+    def table(self):
+        return shell_table(self.md, self.num, self.den, self.partial if self.partials else None,
+                           self.shell_count, self.nframes, self.natoms, self.counts,
+                           self.faber_ziman)
+
+    def column_labels(self):
+        out = []
+        for i in range(len(self.species_type)):
+            for j in range(i, len(self.species_type)):
+                ti, tj = int(self.species_type[i]), int(self.species_type[j])
+                if ti in self.labels and tj in self.labels:
+                    pair = "%s-%s" % (self.labels[ti], self.labels[tj])
+                else:
+                    pair = "%d-%d" % (ti, tj)
+                out.append(("A(" if self.faber_ziman else "S(") + pair + ")")
+        return out
+
+
+def shell_table(md, num, den, partial, shell_count, nframes, natoms, counts, faber_ziman=False):
+    """Rows (q, S, partial columns...), src/sqc_structure_factor.f90:657-719."""
+    nq = md["nq"]
+    dq = (md["qmax"] - md["qmin"]) / nq
+    frames = float(max(nframes, 1))
+    q = md["qmin"] + (np.arange(1, nq + 1) - 0.5) * dq
+    s = np.where(den > 0.0, num / (frames * np.where(den > 0, den, 1.0)), 0.0)
+    cols = [q, s]
+    if partial is not None:
+        S = partial.shape[0]
+        for a in range(S):
+            for b in range(a, S):
+                v = partial[a, b] / (frames * natoms * np.maximum(shell_count, 1))
+                if faber_ziman:
+                    xa, xb = counts[a] / natoms, counts[b] / natoms
+                    v = (v - (xa if a == b else 0.0)) / (xa * xb) + 1.0
+                cols.append(v)
+    return np.stack(cols, axis=1)
+
+
+def format_table(rows, labels):
+    """Text of the shell table as sf_write_results writes it."""
+    out = []
+    if labels:
+        out.append("# q S(q)" + "".join(" " + lab.ljust(24) for lab in labels))
+    else:
+        out.append("# q S(q)")
+    for r in rows:
+        line = "%14.6f  %20.12E" % (r[0], r[1])
+        for v in r[2:]:
+            line += "  %20.12E" % v
+        out.append(line)
+    return "\n".join(out) + "\n"
+
+
+# --------------------------------------------------------------------------
+# NUFFT pieces shared by the CPU restatement
+# --------------------------------------------------------------------------
+
+def next235even(n):
+    """Smallest even 2,3,5-smooth integer >= n (external/finufft/src/finufft_core.cpp:87-90)."""
+    n = max(int(n), 2)
+    if n % 2:
+        n += 1
+    while True:
+        k = n
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return n
+        n += 2
+
+
+def es_params(eps, upsampfac):
+    """(ns, beta) of the ES kernel, rule of external/finufft/src/cuda/spreadinterp.cpp:61-82."""
+    if upsampfac == 2.0:
+        ns = math.ceil(-math.log10(eps / 10.0))
+    else:
+        ns = math.ceil(-math.log(eps) / (math.pi * math.sqrt(1.0 - 1.0 / upsampfac)))
+    ns = max(2, min(ns, 16))
+    bon = 2.30
+    if ns == 2:
+        bon = 2.20
+    if ns == 3:
+        bon = 2.26
+    if ns == 4:
+        bon = 2.38
+    if upsampfac != 2.0:
+        bon = 0.97 * math.pi * (1.0 - 1.0 / (2.0 * upsampfac))
+    return ns, bon * ns
+
+
+def phihat(nf, hmax, ns, beta, nquad=None):
+    """phihat(h) = int phi(2u/ns) cos(2 pi h u / nf) du, h = 0..hmax (Gauss-Legendre)."""
+    nquad = nquad or max(4 * ns, 64)
+    x, w = np.polynomial.legendre.leggauss(nquad)
+    u = 0.5 * ns * x
+    wu = 0.5 * ns * w
+    ker = np.exp(beta * (np.sqrt(1.0 - x * x) - 1.0))
+    h = np.arange(hmax + 1)[:, None]
+    return (wu * ker * np.cos(2.0 * np.pi * h * u[None, :] / nf)).sum(axis=1)
+
+
+def nufft_frame(run, pos, eps=1e-9, upsampfac=1.25, nthreads=None, fields=None):
+    """One frame through the CPU restatement of the reference FINUFFT path.
+
+    Returns (num, partial) for the frame.  The FFT is scipy.fft (pocketfft),
+    sign +1, unnormalised, with ``workers=nthreads``.
+    """
+    import scipy.fft
+    L = lib()
+    md = run.md
+    ns, beta = es_params(eps, upsampfac)
+    n = np.array([next235even(max(math.ceil(upsampfac * m), 2 * ns)) for m in md["m"]],
+                 dtype=np.int64)
+    S = len(run.species_type)
+    per_species = run.partials or run.weight == "xray"
+    nf = S if per_species else 1
+    field_of = run.species_of if per_species else np.zeros(run.natoms, dtype=np.int32)
+    weight = np.ones(run.natoms) if per_species else run.amp[run.species_of]
+    a = _f64(run.cell.ravel(order="F"))
+    grid = np.zeros((nf, n[2], n[1], n[0], 2))
+    rc = L.sqo_nufft_spread(run.natoms, _p(_f64(pos), _d), _p(a, _d), _p(run.origin, _d),
+                            _p(np.ascontiguousarray(field_of, dtype=np.int32), _i32),
+                            _p(_f64(weight), _d), nf, _p(n, _i64), ns, beta, 1000,
+                            _p(grid, _d))
+    if rc:
+        raise MemoryError("cpu nufft spread")
+    g = grid[..., 0] + 1j * grid[..., 1]
+    del grid
+    ghat = scipy.fft.ifftn(g, axes=(1, 2, 3), norm="forward",
+                           workers=nthreads or os.cpu_count())
+    del g
+    gh = np.ascontiguousarray(np.stack([ghat.real, ghat.imag], axis=-1))
+    del ghat
+    hm = [int(np.abs(md["hkl"][:, d]).max()) for d in range(3)]
+    ph = [phihat(int(n[d]), hm[d], ns, beta) for d in range(3)]
+    num = np.zeros(md["nq"])
+    part = np.zeros((nf, nf, md["nq"]))
+    famp = run.amp if per_species else np.ones(1)
+    kind = 2 if run.weight == "xray" else 0
+    L.sqo_nufft_bin(nf, _p(gh, _d), _p(n, _i64), len(md["qlen"]), _p(md["hkl"], _i32),
+                    _p(md["qlen"], _d), _p(md["shell"], _i32), md["nq"], _p(ph[0], _d),
+                    _p(ph[1], _d), _p(ph[2], _d), kind, _p(_f64(famp), _d),
+                    _p(_f64(run.xcoef), _d), _p(num, _d),
+                    _p(part, _d) if run.partials else None)
+    return num, (part if run.partials else None), {"n": n, "ns": ns, "beta": beta}
