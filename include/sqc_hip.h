@@ -1,0 +1,175 @@
+/*
+ * sqc_hip.h -- C ABI of the MI355X (gfx950) reciprocal-space S(q) library.
+ *
+ * This library is the drop-in replacement for the reference's GPU plug-in
+ * (src/sqc_gpu.f90, cufinufft_structure_factor_t) and the cuFINUFFT binding
+ * below it (src/sqc_cufinufft.f90).  Instead of handing a transform plan and
+ * raw device buffers to the host, it owns the whole per-frame path: wrap and
+ * bin the atoms, spread onto the upsampled grid, the 3D FFT, deconvolution,
+ * species combination and the FP64 shell reduction.  The host downloads only
+ * the per-shell sums.
+ *
+ * Conventions
+ *   - All arrays are host memory unless a name says "device".
+ *   - cell[9] is column-major with the lattice vectors a1, a2, a3 in its
+ *     columns, exactly the layout of cell_t%a (src/sqc_cell.f90:17-31).
+ *   - Species are the LAMMPS type ids present in the first frame, ascending
+ *     (src/sqc_structure_factor.f90:1127-1144); species_of[] is 0-based.
+ *   - Shell arrays are [nq], 0-based shell s = reference shell s+1.
+ *   - Pair arrays are [S][S][nq]; only a <= b is filled, matching
+ *     partial_num(t_a, t_b, :) of src/sqc_structure_factor.f90:481-521.
+ *   - Return value: SQC_HIP_OK or an error code; sqc_hip_error_string()
+ *     gives the text and sqc_hip_last_error() a message with detail.
+ *     The library never exits the process.
+ *   - One plan is driven by one host thread at a time.  All device work of a
+ *     plan runs on the plan's own stream; the fetch and gradient calls are
+ *     the synchronisation points.
+ */
+#ifndef SQC_HIP_H
+#define SQC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQC_HIP_ABI_VERSION 1
+
+/* Return codes.  Values 1..20 keep the meaning of FINUFFT's codes where one
+ * applies (external/finufft/include/finufft_errors.h:5-31). */
+enum {
+  SQC_HIP_OK = 0,
+  SQC_HIP_WARN_EPS_TOO_SMALL = 1,   /* eps clipped to what the kernel width allows */
+  SQC_HIP_ERR_MAXNALLOC = 2,        /* grid too large */
+  SQC_HIP_ERR_ALLOC = 11,           /* device or host allocation failed */
+  SQC_HIP_ERR_BAD_ARG = 20,         /* invalid argument (message has detail) */
+  SQC_HIP_ERR_ABI = 30,             /* config struct from another ABI version */
+  SQC_HIP_ERR_NO_DEVICE = 31,       /* no usable HIP device */
+  SQC_HIP_ERR_HIP = 32,             /* a HIP runtime call failed */
+  SQC_HIP_ERR_NO_MODES = 33         /* no reciprocal lattice point in [qmin, qmax] */
+};
+
+enum { SQC_HIP_F64 = 0, SQC_HIP_F32 = 1 };
+enum { SQC_HIP_WEIGHT_UNIT = 0, SQC_HIP_WEIGHT_NEUTRON = 1, SQC_HIP_WEIGHT_XRAY = 2 };
+enum { SQC_HIP_NORM_MEAN = 1, SQC_HIP_NORM_SELF = 2, SQC_HIP_NORM_NATOM = 3 };
+
+/* Plan configuration (plain data; the pointers are read during
+ * sqc_hip_plan_create only). */
+typedef struct sqc_hip_config {
+  int32_t abi_version;      /* must be SQC_HIP_ABI_VERSION */
+  int32_t device;           /* HIP device ordinal (--gpu-id) */
+  int32_t precision;        /* SQC_HIP_F64 or SQC_HIP_F32 (--precision) */
+  int32_t weight_kind;      /* SQC_HIP_WEIGHT_* (-w) */
+  int32_t norm;             /* SQC_HIP_NORM_* (--norm) */
+  int32_t partials;         /* accumulate Re(rho_a rho_b*) per shell (--partials) */
+  int32_t want_grid;        /* accumulate |rho|^2 per kept mode (--grid) */
+  int32_t nq;               /* number of shells (--nq) */
+  double qmin, qmax;        /* shell range (--qmin, --qmax) */
+  double eps;               /* NUFFT tolerance (--eps) */
+  double upsampfac;         /* 0 = automatic (2.0 for f64, 1.25 for f32) */
+  double cell[9];           /* lattice vectors in columns */
+  double origin[3];         /* lower corner of the cell */
+  int64_t natoms;
+  int32_t nspecies;
+  int32_t reserved0;
+  const int32_t *species_of;  /* [natoms], 0-based species of each atom */
+  const double *amp_const;    /* [nspecies] unit/neutron amplitudes */
+  const double *xray_coef;    /* [nspecies][9] IT92 a1..a4, b1..b4, c */
+  int32_t tune;             /* 1 = run the spreader/tile autotuner at plan time */
+  int32_t max_batch;        /* frames per batched launch; 0 = default */
+} sqc_hip_config;
+
+typedef struct sqc_hip_plan sqc_hip_plan;
+
+/* Plan geometry and sizes, for reports and for callers that size buffers. */
+typedef struct sqc_hip_info {
+  int64_t modes[3];         /* m_i, src/sqc_structure_factor.f90:329-338 */
+  int64_t nmodes;           /* kept modes K (both halves) */
+  int64_t nhalf;            /* kept modes with h in the canonical half space */
+  int64_t fine[3];          /* upsampled grid n_i */
+  int32_t nspread;          /* kernel width ns */
+  int32_t nfields;          /* transforms per frame (species, or 1 weighted) */
+  double beta;              /* ES kernel shape */
+  double upsampfac;         /* sigma actually used */
+  int64_t device_bytes;     /* device memory held by the plan */
+  int64_t nlines;           /* x-lines of the pruned half spectrum */
+  int32_t tile[3];          /* spread tile (x, y) and y-pass column chunk */
+  int32_t ncomp;            /* 1 + species pairs: rows of a per-frame output */
+  int64_t rho_len;          /* complex values per field in a saved rho buffer */
+} sqc_hip_info;
+
+/* Number of visible HIP devices (cudaGetDeviceCount in src/sqc_cufinufft.f90:143-174). */
+int sqc_hip_device_count(int *count);
+
+/* Create a plan: sizes, mode tables, denominators and device buffers
+ * (replaces cufinufft_makeplan, src/sqc_gpu.f90:185-209, and the mode/shell
+ * set-up of sf_configure, src/sqc_structure_factor.f90:313-422). */
+int sqc_hip_plan_create(sqc_hip_plan **plan, const sqc_hip_config *cfg);
+
+int sqc_hip_plan_info(const sqc_hip_plan *plan, sqc_hip_info *info);
+
+/* Shell bookkeeping of the plan (den and shell_count of
+ * src/sqc_structure_factor.f90:408-416). */
+int sqc_hip_shells(const sqc_hip_plan *plan, double *den, int64_t *shell_count);
+
+/* Kept modes in the reference's enumeration order (hkl[3K] int32, qvec[3K],
+ * gidx[K] 1-based), for the --grid table (src/sqc_structure_factor.f90:815-823). */
+int sqc_hip_modes(const sqc_hip_plan *plan, int32_t *hkl, double *qvec, int64_t *gidx);
+
+/* Accumulate nframes frames.  pos holds Cartesian positions, (3, natoms)
+ * column-major per frame, frames consecutive (frame_t%pos of
+ * src/sqc_dump.f90:22-45).  Host memory; copied before the call returns.
+ * Replaces gpu_accumulate (src/sqc_gpu.f90:213-362). */
+int sqc_hip_accumulate(sqc_hip_plan *plan, const double *pos, int64_t nframes);
+
+/* Same with the positions already in device memory (HBM-resident input). */
+int sqc_hip_accumulate_device(sqc_hip_plan *plan, const double *pos_device, int64_t nframes);
+
+/* Synchronise and copy out the accumulators: num[nq], partial[S][S][nq]
+ * (may be NULL), gnum[K] in reference mode order (may be NULL), nframes. */
+int sqc_hip_fetch(sqc_hip_plan *plan, double *num, double *partial, double *gnum,
+                  int64_t *nframes);
+
+/* Zero the accumulators. */
+int sqc_hip_reset(sqc_hip_plan *plan);
+
+/* Per-frame forward on device memory, for the autograd layer.  For each of
+ * nframes frames writes, on the plan's stream:
+ *   out_device[f][1 + npairs][nq]  num, then the a<=b pair sums
+ *   rho_device (optional) the species amplitudes at the canonical kept modes,
+ *              [f][nfields][rho_len] complex of the plan precision.
+ * stream: a hipStream_t to order against (NULL = plan stream only). */
+int sqc_hip_forward_device(sqc_hip_plan *plan, const double *pos_device, int64_t nframes,
+                           double *out_device, void *rho_device, void *stream);
+
+/* Adjoint on device memory: given the upstream gradients of one frame's
+ * normalised shell values,
+ *   gS_device[nq]            dL/dS_s,      S_s  = num_s / den_s
+ *   gP_device[S][S][nq]      dL/dP_ab,s,   P_ab = pair_ab,s / (N count_s)  (may be NULL)
+ * and the rho saved by sqc_hip_forward_device for the same positions, writes
+ * grad_device[3*natoms] = dL/dr (FP64).  No counterpart in the reference. */
+int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const void *rho_device,
+                           const double *gS_device, const double *gP_device,
+                           double *grad_device, void *stream);
+
+/* Convenience: loss and gradient of L = sum_s w_s (S_s - T_s)^2 for one
+ * frame, host memory in and out (target[nq], wq[nq], grad[3*natoms]). */
+int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target,
+                     const double *wq, double *loss, double *grad);
+
+int sqc_hip_destroy(sqc_hip_plan *plan);
+
+const char *sqc_hip_error_string(int code);
+const char *sqc_hip_last_error(void);
+
+/* sizeof(sqc_hip_config), for the host-side layout check that mirrors the
+ * reference's opts ABI check (src/sqc_cufinufft.f90:277-290). */
+size_t sqc_hip_config_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SQC_HIP_H */

@@ -1,0 +1,106 @@
+"""ctypes binding of libsqc_hip.so (declarations in include/sqc_hip.h).
+
+The product path always goes through this library; there is no CPU
+fallback.  ``load()`` raises when the shared object is missing.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsqc_hip.so")
+
+ABI_VERSION = 1
+F64, F32 = 0, 1
+WEIGHT = {"unit": 0, "neutron": 1, "xray": 2}
+NORM = {"mean": 1, "self": 2, "n": 3}
+
+_c = ctypes
+_vp = _c.c_void_p
+
+
+class Config(_c.Structure):
+    _fields_ = [
+        ("abi_version", _c.c_int32), ("device", _c.c_int32), ("precision", _c.c_int32),
+        ("weight_kind", _c.c_int32), ("norm", _c.c_int32), ("partials", _c.c_int32),
+        ("want_grid", _c.c_int32), ("nq", _c.c_int32),
+        ("qmin", _c.c_double), ("qmax", _c.c_double), ("eps", _c.c_double),
+        ("upsampfac", _c.c_double), ("cell", _c.c_double * 9), ("origin", _c.c_double * 3),
+        ("natoms", _c.c_int64), ("nspecies", _c.c_int32), ("reserved0", _c.c_int32),
+        ("species_of", _c.POINTER(_c.c_int32)), ("amp_const", _c.POINTER(_c.c_double)),
+        ("xray_coef", _c.POINTER(_c.c_double)), ("tune", _c.c_int32),
+        ("max_batch", _c.c_int32),
+    ]
+
+
+class Info(_c.Structure):
+    _fields_ = [
+        ("modes", _c.c_int64 * 3), ("nmodes", _c.c_int64), ("nhalf", _c.c_int64),
+        ("fine", _c.c_int64 * 3), ("nspread", _c.c_int32), ("nfields", _c.c_int32),
+        ("beta", _c.c_double), ("upsampfac", _c.c_double), ("device_bytes", _c.c_int64),
+        ("nlines", _c.c_int64), ("tile", _c.c_int32 * 3), ("ncomp", _c.c_int32),
+        ("rho_len", _c.c_int64),
+    ]
+
+
+# Every entry point of include/sqc_hip.h with its ctypes signature.
+SIGNATURES = {
+    "sqc_hip_device_count": (_c.c_int, [_c.POINTER(_c.c_int)]),
+    "sqc_hip_plan_create": (_c.c_int, [_c.POINTER(_vp), _c.POINTER(Config)]),
+    "sqc_hip_plan_info": (_c.c_int, [_vp, _c.POINTER(Info)]),
+    "sqc_hip_shells": (_c.c_int, [_vp, _vp, _vp]),
+    "sqc_hip_modes": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "sqc_hip_accumulate": (_c.c_int, [_vp, _vp, _c.c_int64]),
+    "sqc_hip_accumulate_device": (_c.c_int, [_vp, _vp, _c.c_int64]),
+    "sqc_hip_fetch": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "sqc_hip_reset": (_c.c_int, [_vp]),
+    "sqc_hip_forward_device": (_c.c_int, [_vp, _vp, _c.c_int64, _vp, _vp, _vp]),
+    "sqc_hip_adjoint_device": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sqc_hip_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sqc_hip_destroy": (_c.c_int, [_vp]),
+    "sqc_hip_error_string": (_c.c_char_p, [_c.c_int]),
+    "sqc_hip_last_error": (_c.c_char_p, []),
+    "sqc_hip_config_size": (_c.c_size_t, []),
+}
+
+_LIB = None
+
+
+class SqcHipError(RuntimeError):
+    def __init__(self, code, where):
+        lib = load()
+        msg = lib.sqc_hip_last_error().decode() or lib.sqc_hip_error_string(code).decode()
+        super().__init__("%s failed (%d): %s" % (where, code, msg))
+        self.code = code
+
+
+def load():
+    """Load libsqc_hip.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsqc_hip.so is not built (%s); run __graft_entry__.build() or "
+                           "python -m sqcalc_amd.build" % LIB_PATH)
+    lib = _c.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.sqc_hip_config_size() != _c.sizeof(Config):
+        raise RuntimeError("sqc_hip_config layout mismatch: library %d bytes, binding %d"
+                           % (lib.sqc_hip_config_size(), _c.sizeof(Config)))
+    _LIB = lib
+    return lib
+
+
+def check(code, where, warn_ok=True):
+    if code == 0 or (warn_ok and code == 1):
+        return code
+    raise SqcHipError(code, where)
+
+
+def device_count():
+    n = _c.c_int(0)
+    rc = load().sqc_hip_device_count(_c.byref(n))
+    return n.value if rc == 0 else 0

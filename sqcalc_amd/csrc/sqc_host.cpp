@@ -1,0 +1,921 @@
+// sqc_host.cpp -- plan construction and the C ABI of libsqc_hip.
+//
+// Plan creation restates, for the GPU library, what the reference computes on
+// the host before the first frame:
+//   cell set-up, B = 2 pi A^-T              src/sqc_cell.f90:76-98
+//   m_i, kept modes, shells, den, counts    src/sqc_structure_factor.f90:327-451
+//   kernel width / shape, fine grid         external/finufft/src/cuda/spreadinterp.cpp:61-82,
+//                                           external/finufft/src/finufft_core.cpp:81-99
+//   kernel Fourier coefficients             external/finufft/src/cuda/common.cu:27-66
+// and then lays the kept modes out for the device: x-lines of the canonical
+// half spectrum, a u16 shell per candidate mode, and the tile shapes of the
+// three DFT passes.  Every |q| and shell index is evaluated with the
+// reference's expression order and FP contraction off, so the shell of each
+// mode is the reference's, bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sqc_internal.hpp"
+
+#pragma clang fp contract(off)
+
+struct sqc_hip_plan {
+  sqc::Plan p;
+  hipStream_t own_stream = nullptr;
+  // scratch of the host-memory convenience entry points
+  double *g_pos = nullptr, *g_out = nullptr, *g_gs = nullptr, *g_grad = nullptr;
+  void *g_rho = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return SQC_HIP_ERR_HIP;
+}
+
+#define HIP_TRY(call, what)                          \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+double det3(const double *m) {
+  auto M = [&](int i, int j) { return m[i + 3 * j]; };
+  return M(0, 0) * (M(1, 1) * M(2, 2) - M(1, 2) * M(2, 1)) -
+         M(0, 1) * (M(1, 0) * M(2, 2) - M(1, 2) * M(2, 0)) +
+         M(0, 2) * (M(1, 0) * M(2, 1) - M(1, 1) * M(2, 0));
+}
+
+// Cofactor inverse in the reference's term order (src/sqc_cell.f90:135-151).
+void inverse3(const double *m, double *inv) {
+  auto M = [&](int i, int j) { return m[i + 3 * j]; };
+  auto I = [&](int i, int j) -> double & { return inv[i + 3 * j]; };
+  const double det = det3(m);
+  I(0, 0) = (M(1, 1) * M(2, 2) - M(1, 2) * M(2, 1));
+  I(1, 0) = -(M(1, 0) * M(2, 2) - M(1, 2) * M(2, 0));
+  I(2, 0) = (M(1, 0) * M(2, 1) - M(1, 1) * M(2, 0));
+  I(0, 1) = -(M(0, 1) * M(2, 2) - M(0, 2) * M(2, 1));
+  I(1, 1) = (M(0, 0) * M(2, 2) - M(0, 2) * M(2, 0));
+  I(2, 1) = -(M(0, 0) * M(2, 1) - M(0, 1) * M(2, 0));
+  I(0, 2) = (M(0, 1) * M(1, 2) - M(0, 2) * M(1, 1));
+  I(1, 2) = -(M(0, 0) * M(1, 2) - M(0, 2) * M(1, 0));
+  I(2, 2) = (M(0, 0) * M(1, 1) - M(0, 1) * M(1, 0));
+  for (int k = 0; k < 9; ++k) inv[k] = inv[k] / det;
+}
+
+struct ModeGeom {
+  double b[9];
+  double qmin, qmax, dq;
+  int nq;
+  // |q| of h with the reference's order: q_i = (b1_i h1 + b2_i h2) + b3_i h3
+  double qlen(int h1, int h2, int h3, double *q = nullptr) const {
+    const double r1 = h1, r2 = h2, r3 = h3;
+    double v[3];
+    for (int i = 0; i < 3; ++i) v[i] = b[i] * r1 + b[i + 3] * r2 + b[i + 6] * r3;
+    if (q) {
+      q[0] = v[0];
+      q[1] = v[1];
+      q[2] = v[2];
+    }
+    return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  }
+  bool kept(double ql) const { return !(ql > qmax || ql < qmin); }
+  int shell(double ql) const {  // 0-based
+    int s = (int)((ql - qmin) / dq) + 1;
+    if (s < 1) s = 1;
+    if (s > nq) s = nq;
+    return s - 1;
+  }
+};
+
+// Smallest even 2,3,5-smooth integer >= n (external/finufft/src/finufft_core.cpp:81-99).
+int64_t next235even(int64_t n) {
+  if (n <= 2) return 2;
+  if (n % 2) ++n;
+  for (;; n += 2) {
+    int64_t k = n;
+    for (int p : {2, 3, 5})
+      while (k % p == 0) k /= p;
+    if (k == 1) return n;
+  }
+}
+
+bool factor(int n, sqc::AxisPlan &ax) {
+  ax.n = n;
+  ax.nst = 0;
+  for (int r : {8, 5, 4, 3, 2}) {
+    while (n % r == 0) {
+      if (ax.nst >= sqc::kMaxStages) return false;
+      ax.rad[ax.nst++] = r;
+      n /= r;
+    }
+  }
+  return n == 1;
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1] by Newton iteration on P_q.
+void gauss_legendre(int q, std::vector<double> &x, std::vector<double> &w) {
+  x.assign(q, 0.0);
+  w.assign(q, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < (q + 1) / 2; ++i) {
+    double z = std::cos(pi * (i + 0.75) / (q + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= q; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = q * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = -z;
+    x[q - 1 - i] = z;
+    w[i] = w[q - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+// phihat(h) = int phi(2u/ns) cos(2 pi h u / nf) du for h = 0..hmax.
+std::vector<double> phihat(int nf, int hmax, int ns, double beta) {
+  std::vector<double> x, w;
+  gauss_legendre(std::max(4 * ns, 64), x, w);
+  const double pi = 3.14159265358979323846;
+  std::vector<double> out(hmax + 1, 0.0);
+  for (int h = 0; h <= hmax; ++h) {
+    double s = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) {
+      const double u = 0.5 * ns * x[i];
+      const double ker = std::exp(beta * (std::sqrt(1.0 - x[i] * x[i]) - 1.0));
+      s += 0.5 * ns * w[i] * ker * std::cos(2.0 * pi * h * u / nf);
+    }
+    out[h] = s;
+  }
+  return out;
+}
+
+// exp(+2 pi i t / n) for t = 0..n-1, using the symmetry of the circle so that
+// every entry comes from an angle in [0, pi/4].
+void twiddles(int n, std::vector<double> &re, std::vector<double> &im) {
+  re.resize(n);
+  im.resize(n);
+  const double pi = 3.14159265358979323846;
+  for (int t = 0; t < n; ++t) {
+    // reduce t/n to an octant: angle = 2 pi t / n
+    const long long k8 = 8LL * t;  // octant index * n
+    const int oct = (int)(k8 / n);
+    const double rem = (double)(k8 - (long long)oct * n) / n;  // in [0,1)
+    double a = rem * pi / 4.0;
+    double c, s;
+    // angle = oct * pi/4 + a
+    switch (oct) {
+      case 0: c = std::cos(a); s = std::sin(a); break;
+      case 1: c = std::sin(pi / 4.0 - a); s = std::cos(pi / 4.0 - a); break;
+      case 2: c = -std::sin(a); s = std::cos(a); break;
+      case 3: c = -std::cos(pi / 4.0 - a); s = std::sin(pi / 4.0 - a); break;
+      case 4: c = -std::cos(a); s = -std::sin(a); break;
+      case 5: c = -std::sin(pi / 4.0 - a); s = -std::cos(pi / 4.0 - a); break;
+      case 6: c = std::sin(a); s = -std::cos(a); break;
+      default: c = std::cos(pi / 4.0 - a); s = -std::sin(pi / 4.0 - a); break;
+    }
+    re[t] = c;
+    im[t] = s;
+  }
+}
+
+template <typename T>
+int upload(T **dst, const void *src, size_t bytes, int64_t &total) {
+  const size_t nb = bytes ? bytes : 8;
+  hipError_t e = hipMalloc((void **)dst, nb);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  total += (int64_t)nb;
+  if (src && bytes) {
+    e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+  }
+  return 0;
+}
+
+template <typename T>
+int alloc(T **dst, size_t bytes, int64_t &total) {
+  return upload(dst, nullptr, bytes, total);
+}
+
+int upload_twiddles(sqc::Plan &p, int axis) {
+  std::vector<double> re, im;
+  twiddles(p.geo.n[axis], re, im);
+  const int n = p.geo.n[axis];
+  if (p.prec == 1) {
+    std::vector<float> buf(2 * n);
+    for (int t = 0; t < n; ++t) {
+      buf[2 * t] = (float)re[t];
+      buf[2 * t + 1] = (float)im[t];
+    }
+    return upload((float **)&p.d_tw[axis], buf.data(), buf.size() * sizeof(float), p.bytes);
+  }
+  std::vector<double> buf(2 * n);
+  for (int t = 0; t < n; ++t) {
+    buf[2 * t] = re[t];
+    buf[2 * t + 1] = im[t];
+  }
+  return upload((double **)&p.d_tw[axis], buf.data(), buf.size() * sizeof(double), p.bytes);
+}
+
+double amplitude(const sqc::Geo &g, int sp, double q) {
+  if (g.weight_kind == SQC_HIP_WEIGHT_XRAY) {
+    const double *c = g.xcoef[sp];
+    double st = q / (4.0 * std::acos(-1.0));
+    st = st * st;
+    double f = c[8];
+    for (int i = 0; i < 4; ++i) f = f + c[i] * std::exp(-c[4 + i] * st);
+    return f;
+  }
+  return g.amp[sp];
+}
+
+void free_plan(sqc_hip_plan *h) {
+  sqc::Plan &p = h->p;
+  void *bufs[] = {p.d_pos, p.d_X, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
+                  p.d_species, p.d_W1, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
+                  p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
+                  p.d_line_off, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
+                  p.d_gnum, p.d_slots, p.d_gs, p.d_rho, h->g_pos, h->g_out, h->g_gs,
+                  h->g_grad, h->g_rho};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+}
+
+int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
+  sqc::Plan &p = h->p;
+  sqc::Geo &g = p.geo;
+  p.cfg = *cfg;
+  p.prec = cfg->precision;
+  const int S = cfg->nspecies;
+  g.S = S;
+  g.natoms = cfg->natoms;
+  g.nq = cfg->nq;
+  g.weight_kind = cfg->weight_kind;
+  g.partials = cfg->partials ? 1 : 0;
+  for (int a = 0; a < S; ++a) {
+    g.amp[a] = cfg->weight_kind == SQC_HIP_WEIGHT_UNIT ? 1.0
+               : cfg->amp_const ? cfg->amp_const[a] : 1.0;
+    for (int i = 0; i < 9; ++i) g.xcoef[a][i] = cfg->xray_coef ? cfg->xray_coef[9 * a + i] : 0.0;
+  }
+  p.species_of.assign(cfg->species_of, cfg->species_of + cfg->natoms);
+  p.counts.assign(S, 0);
+  for (int64_t j = 0; j < cfg->natoms; ++j) {
+    const int s = p.species_of[j];
+    if (s < 0 || s >= S) return fail(SQC_HIP_ERR_BAD_ARG, "species_of out of range");
+    ++p.counts[s];
+  }
+  // fields: one weighted transform unless the pairs or q-dependent weights
+  // need each species on its own (src/sqc_gpu.f90:193-197)
+  const bool per_species = cfg->partials || cfg->weight_kind == SQC_HIP_WEIGHT_XRAY;
+  g.F = per_species ? S : 1;
+  g.ncomp = 1 + ((cfg->partials && S > 0) ? S * (S + 1) / 2 : 0);
+
+  // --- cell and modes ---
+  const double two_pi = 2.0 * std::acos(-1.0);
+  inverse3(cfg->cell, g.ainv);
+  ModeGeom mg{};
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) mg.b[i + 3 * j] = two_pi * g.ainv[j + 3 * i];
+  std::memcpy(g.b, mg.b, sizeof(g.b));
+  std::memcpy(g.origin, cfg->origin, sizeof(g.origin));
+  mg.qmin = cfg->qmin;
+  mg.qmax = cfg->qmax;
+  mg.nq = cfg->nq;
+  mg.dq = (cfg->qmax - cfg->qmin) / (double)cfg->nq;
+  if (!(mg.dq > 0.0)) return fail(SQC_HIP_ERR_BAD_ARG, "q range must be positive");
+  p.dq = mg.dq;
+  for (int i = 0; i < 3; ++i) {
+    const double *a = cfg->cell + 3 * i;
+    const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if (!(len > 0.0)) return fail(SQC_HIP_ERR_BAD_ARG, "degenerate simulation cell");
+    int64_t mi = 2 * (int64_t)std::ceil(cfg->qmax * len / two_pi) + 1;
+    p.m[i] = std::max<int64_t>(mi, 3);
+    g.M[i] = (int)((p.m[i] - 1) / 2);
+  }
+  if (p.m[0] * p.m[1] * p.m[2] > 400000000LL)
+    return fail(SQC_HIP_ERR_MAXNALLOC, "reciprocal grid exceeds 4e8 points; reduce qmax");
+
+  // denominators and counts over every kept mode, reference order
+  p.den.assign(cfg->nq, 0.0);
+  p.shell_count.assign(cfg->nq, 0);
+  p.nmodes = 0;
+  for (int64_t p3 = 1; p3 <= p.m[2]; ++p3) {
+    const int h3 = (int)(p3 - (p.m[2] + 1) / 2);
+    for (int64_t p2 = 1; p2 <= p.m[1]; ++p2) {
+      const int h2 = (int)(p2 - (p.m[1] + 1) / 2);
+      for (int64_t p1 = 1; p1 <= p.m[0]; ++p1) {
+        const int h1 = (int)(p1 - (p.m[0] + 1) / 2);
+        if (h1 == 0 && h2 == 0 && h3 == 0) continue;
+        const double ql = mg.qlen(h1, h2, h3);
+        if (!mg.kept(ql)) continue;
+        ++p.nmodes;
+        const int s = mg.shell(ql);
+        double d;
+        if (cfg->norm == SQC_HIP_NORM_NATOM) {
+          d = (double)cfg->natoms;
+        } else {
+          double sw = 0.0, sw2 = 0.0;
+          for (int t = 0; t < S; ++t) {
+            if (p.counts[t] <= 0) continue;
+            const double w = amplitude(g, t, ql);
+            sw = sw + (double)p.counts[t] * w;
+            sw2 = sw2 + (double)p.counts[t] * w * w;
+          }
+          d = cfg->norm == SQC_HIP_NORM_MEAN ? sw * sw / (double)cfg->natoms : sw2;
+        }
+        p.den[s] += d;
+        p.shell_count[s] += 1;
+      }
+    }
+  }
+  if (p.nmodes == 0)
+    return fail(SQC_HIP_ERR_NO_MODES, "no reciprocal lattice points fall inside the requested q range");
+
+  // canonical half-space lines: h3 > 0; h3 = 0, h2 > 0; h3 = h2 = 0, h1 > 0
+  const int M1 = g.M[0], M2 = g.M[1], M3 = g.M[2];
+  p.lo2.assign(M3 + 1, 0);
+  p.hi2.assign(M3 + 1, -1);
+  p.loff.assign(M3 + 1, 0);
+  p.line_info.clear();
+  p.line_off.clear();
+  std::vector<uint16_t> stab;
+  p.nhalf = 0;
+  int nlines = 0;
+  for (int h3 = 0; h3 <= M3; ++h3) {
+    const int h2a = (h3 == 0) ? 0 : -M2;
+    // first pass: which h2 have kept modes
+    int lo = 1 << 30, hi = -(1 << 30);
+    std::vector<int> l1(2 * M2 + 1, 1 << 30), h1v(2 * M2 + 1, -(1 << 30));
+    for (int h2 = h2a; h2 <= M2; ++h2) {
+      const int h1a = (h3 == 0 && h2 == 0) ? 1 : -M1;
+      for (int h1 = h1a; h1 <= M1; ++h1) {
+        if (mg.kept(mg.qlen(h1, h2, h3))) {
+          l1[h2 + M2] = std::min(l1[h2 + M2], h1);
+          h1v[h2 + M2] = std::max(h1v[h2 + M2], h1);
+        }
+      }
+      if (h1v[h2 + M2] >= l1[h2 + M2]) {
+        lo = std::min(lo, h2);
+        hi = std::max(hi, h2);
+      }
+    }
+    p.loff[h3] = nlines;
+    if (hi < lo) {
+      p.lo2[h3] = 0;
+      p.hi2[h3] = -1;
+      continue;
+    }
+    p.lo2[h3] = lo;
+    p.hi2[h3] = hi;
+    for (int h2 = lo; h2 <= hi; ++h2) {
+      int a = l1[h2 + M2], b = h1v[h2 + M2];
+      if (b < a) { a = 0; b = -1; }
+      p.line_info.insert(p.line_info.end(), {h2, h3, a, b});
+      p.line_off.push_back((int32_t)stab.size());
+      for (int h1 = a; h1 <= b; ++h1) {
+        const double ql = mg.qlen(h1, h2, h3);
+        const bool kp = mg.kept(ql) && !(h1 == 0 && h2 == 0 && h3 == 0);
+        stab.push_back(kp ? (uint16_t)mg.shell(ql) : sqc::kNoShell);
+        if (kp) ++p.nhalf;
+      }
+      if (stab.size() > 2000000000ULL)
+        return fail(SQC_HIP_ERR_MAXNALLOC, "too many candidate modes");
+      ++nlines;
+    }
+  }
+  p.line_off.push_back((int32_t)stab.size());
+  p.ncand = (int64_t)stab.size();
+  g.nlines = nlines;
+  if (2 * p.nhalf != p.nmodes)
+    return fail(SQC_HIP_ERR_BAD_ARG, "kept-mode set is not symmetric under h -> -h");
+
+  // --- kernel and fine grid ---
+  double eps = cfg->eps > 0 ? cfg->eps : (p.prec == 1 ? 1e-6 : 1e-9);
+  double sigma = cfg->upsampfac > 0 ? cfg->upsampfac : (p.prec == 1 ? 1.25 : 2.0);
+  if (!(sigma > 1.0)) return fail(SQC_HIP_ERR_BAD_ARG, "upsampfac must exceed 1");
+  p.sigma = sigma;
+  int ret = SQC_HIP_OK;
+  const double emin = p.prec == 1 ? 1.2e-7 : 2.3e-16;
+  if (eps < emin) {
+    eps = emin;
+    ret = SQC_HIP_WARN_EPS_TOO_SMALL;
+  }
+  int ns;
+  if (sigma == 2.0)
+    ns = (int)std::ceil(-std::log10(eps / 10.0));
+  else
+    ns = (int)std::ceil(-std::log(eps) / (3.14159265358979323846 * std::sqrt(1.0 - 1.0 / sigma)));
+  ns = std::max(2, ns);
+  if (ns > 16) {
+    ns = 16;
+    ret = SQC_HIP_WARN_EPS_TOO_SMALL;
+  }
+  double bon = 2.30;
+  if (ns == 2) bon = 2.20;
+  if (ns == 3) bon = 2.26;
+  if (ns == 4) bon = 2.38;
+  if (sigma != 2.0) bon = 0.97 * 3.14159265358979323846 * (1.0 - 1.0 / (2.0 * sigma));
+  g.ns = ns;
+  g.beta = bon * ns;
+  g.half_ns = 0.5 * ns;
+  for (int i = 0; i < 3; ++i) {
+    const int64_t nf = next235even(std::max<int64_t>((int64_t)std::ceil(sigma * p.m[i]), 2 * ns));
+    if (nf > 16384) return fail(SQC_HIP_ERR_MAXNALLOC, "fine grid exceeds 16384 points per axis");
+    g.n[i] = (int)nf;
+    if (!factor(g.n[i], p.ax[i])) return fail(SQC_HIP_ERR_BAD_ARG, "fine grid not 2,3,5-smooth");
+  }
+
+  // --- launch shapes ---
+  const size_t vs = p.prec == 1 ? 8 : 16;
+  const int maxe = p.prec == 1 ? 8192 : 4096;
+  const size_t lds_cap = 160 * 1024;
+  const size_t lds_pref = 80 * 1024;  // two workgroups per CU
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+  {
+    static const int cand[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {4, 2}, {4, 1}, {2, 1}};
+    int pick = -1;
+    for (int pass = 0; pass < 2 && pick < 0; ++pass)
+      for (int c = 0; c < 7; ++c) {
+        const int txr = cand[c][0], ty = cand[c][1];
+        const int cc = (txr / 2) * ty;
+        const size_t lds = ((size_t)cc * n3 + n3) * vs;
+        if (cc * n3 > maxe) continue;
+        if (lds > (pass == 0 ? lds_pref : lds_cap)) continue;
+        if (2 * txr > n1 || 2 * ty > n2) continue;
+        pick = c;
+        break;
+      }
+    if (pick < 0) return fail(SQC_HIP_ERR_MAXNALLOC, "z line too long for the LDS spreader");
+    g.txr = cand[pick][0];
+    g.ty = cand[pick][1];
+    p.lds1 = ((size_t)(g.txr / 2) * g.ty * n3 + n3) * vs;
+  }
+  g.ntx = (n1 + g.txr - 1) / g.txr;
+  g.nty = (n2 + g.ty - 1) / g.ty;
+  g.nbins = g.ntx * g.nty;
+  // a stencil that wraps past a narrow last tile can come from one bin further
+  g.hx = (int)std::ceil((0.5 * ns + 1.0) / g.txr) + (n1 % g.txr ? 1 : 0);
+  g.hy = (int)std::ceil((0.5 * ns + 1.0) / g.ty) + (n2 % g.ty ? 1 : 0);
+  g.sx = std::min(g.ntx, (ns - 1 + g.txr - 1) / g.txr + 2);
+  g.sy = std::min(g.nty, (ns - 1 + g.ty - 1) / g.ty + 2);
+  {
+    int tx2 = 16;
+    while (tx2 > 1 && (tx2 * n2 > maxe || ((size_t)tx2 * n2 + n2) * vs > lds_pref)) tx2 /= 2;
+    if (tx2 * n2 > maxe) return fail(SQC_HIP_ERR_MAXNALLOC, "y line too long for the LDS DFT");
+    p.tx2 = tx2;
+    p.lds2 = ((size_t)tx2 * n2 + n2) * vs;
+  }
+  {
+    const size_t accb = (size_t)g.ncomp * g.nq * sizeof(double);
+    int lb = 8;
+    auto lds3 = [&](int l) { return ((size_t)l * g.F * n1 + n1) * vs + accb; };
+    while (lb > 1 && (lb * g.F * n1 > maxe || lds3(lb) > lds_pref)) --lb;
+    if (lb * g.F * n1 > maxe || lds3(lb) > lds_cap)
+      return fail(SQC_HIP_ERR_MAXNALLOC, "x line x species too large for the LDS DFT");
+    p.lb = lb;
+    p.lds3 = lds3(lb);
+  }
+  int dev_count = 0;
+  HIP_TRY(hipGetDeviceCount(&dev_count), "hipGetDeviceCount");
+  if (cfg->device < 0 || cfg->device >= dev_count)
+    return fail(SQC_HIP_ERR_NO_DEVICE, "HIP device " + std::to_string(cfg->device) +
+                                           " requested but " + std::to_string(dev_count) +
+                                           " device(s) available");
+  p.device = cfg->device;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
+  const int ngroups = (g.nlines + p.lb - 1) / p.lb;
+  p.nblk3 = std::max(1, std::min(ngroups, 2 * prop.multiProcessorCount));
+  HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+  p.stream = h->own_stream;
+
+  // --- device buffers ---
+  const int64_t N = cfg->natoms;
+  const int nbt = g.F * g.nbins;
+  int rc;
+  p.batch_cap = cfg->max_batch > 0 ? cfg->max_batch : 8;
+  if ((rc = alloc(&p.d_pos, sizeof(double) * 3 * N * p.batch_cap, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_X, sizeof(double) * 3 * N, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_key, sizeof(int32_t) * N, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_sidx, sizeof(int32_t) * N, p.bytes))) return rc;
+  if ((rc = upload(&p.d_species, p.species_of.data(), sizeof(int32_t) * N, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_count, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
+  if ((rc = alloc(&p.d_start, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
+  if ((rc = alloc(&p.d_cursor, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
+  HIP_TRY(hipMemset(p.d_count, 0, sizeof(int32_t) * (nbt + 1)), "hipMemset");
+  const size_t w1 = (size_t)g.F * (M3 + 1) * n2 * n1 * vs;
+  const size_t w2 = (size_t)g.F * std::max(g.nlines, 1) * n1 * vs;
+  if ((rc = alloc((char **)&p.d_W1, w1, p.bytes))) return rc;
+  if ((rc = alloc((char **)&p.d_W2, w2, p.bytes))) return rc;
+  for (int i = 0; i < 3; ++i)
+    if ((rc = upload_twiddles(p, i))) return rc;
+  for (int i = 0; i < 3; ++i) {
+    std::vector<double> ph = phihat(g.n[i], g.M[i], ns, g.beta);
+    if ((rc = upload(&p.d_ph[i], ph.data(), ph.size() * sizeof(double), p.bytes))) return rc;
+  }
+  if ((rc = upload(&p.d_lo2, p.lo2.data(), p.lo2.size() * 4, p.bytes))) return rc;
+  if ((rc = upload(&p.d_hi2, p.hi2.data(), p.hi2.size() * 4, p.bytes))) return rc;
+  if ((rc = upload(&p.d_loff, p.loff.data(), p.loff.size() * 4, p.bytes))) return rc;
+  if ((rc = upload(&p.d_line_info, p.line_info.data(), p.line_info.size() * 4, p.bytes))) return rc;
+  if ((rc = upload(&p.d_line_off, p.line_off.data(), p.line_off.size() * 4, p.bytes))) return rc;
+  if ((rc = upload(&p.d_stab, stab.data(), stab.size() * 2, p.bytes))) return rc;
+  if ((rc = upload(&p.d_den, p.den.data(), p.den.size() * 8, p.bytes))) return rc;
+  {
+    std::vector<double> cnt(p.shell_count.begin(), p.shell_count.end());
+    if ((rc = upload(&p.d_cnt, cnt.data(), cnt.size() * 8, p.bytes))) return rc;
+  }
+  const size_t ntot = (size_t)g.ncomp * g.nq;
+  if ((rc = alloc(&p.d_wgpart, sizeof(double) * ntot * p.nblk3, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_acc, sizeof(double) * ntot * 2, p.bytes))) return rc;
+  HIP_TRY(hipMemset(p.d_acc, 0, sizeof(double) * ntot * 2), "hipMemset");
+  if (cfg->want_grid) {
+    if ((rc = alloc(&p.d_gnum, sizeof(double) * p.ncand, p.bytes))) return rc;
+    HIP_TRY(hipMemset(p.d_gnum, 0, sizeof(double) * p.ncand), "hipMemset");
+  }
+  sqc::kernel_lds_limits(p);
+  HIP_TRY(hipDeviceSynchronize(), "plan set-up");
+  return ret;
+}
+
+int check_plan(const sqc_hip_plan *plan) {
+  if (!plan) return fail(SQC_HIP_ERR_BAD_ARG, "null plan");
+  return 0;
+}
+
+struct StreamScope {
+  sqc::Plan &p;
+  hipStream_t saved;
+  StreamScope(sqc::Plan &pl, void *s) : p(pl), saved(pl.stream) {
+    if (s) p.stream = (hipStream_t)s;
+  }
+  ~StreamScope() { p.stream = saved; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int sqc_hip_device_count(int *count) {
+  if (!count) return fail(SQC_HIP_ERR_BAD_ARG, "null count");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return hip_fail(e, "hipGetDeviceCount");
+  }
+  return SQC_HIP_OK;
+}
+
+size_t sqc_hip_config_size(void) { return sizeof(sqc_hip_config); }
+
+int sqc_hip_plan_create(sqc_hip_plan **plan, const sqc_hip_config *cfg) {
+  if (!plan || !cfg) return fail(SQC_HIP_ERR_BAD_ARG, "null argument");
+  *plan = nullptr;
+  if (cfg->abi_version != SQC_HIP_ABI_VERSION)
+    return fail(SQC_HIP_ERR_ABI, "sqc_hip_config from ABI version " +
+                                     std::to_string(cfg->abi_version) + ", library has " +
+                                     std::to_string(SQC_HIP_ABI_VERSION));
+  if (cfg->precision != SQC_HIP_F64 && cfg->precision != SQC_HIP_F32)
+    return fail(SQC_HIP_ERR_BAD_ARG, "precision must be SQC_HIP_F64 or SQC_HIP_F32");
+  if (cfg->natoms <= 0 || cfg->natoms > 2000000000LL || !cfg->species_of)
+    return fail(SQC_HIP_ERR_BAD_ARG, "natoms must be in 1..2e9 with species_of given");
+  if (cfg->nspecies < 1 || cfg->nspecies > sqc::kMaxSpecies)
+    return fail(SQC_HIP_ERR_BAD_ARG, "nspecies must be in 1..8");
+  if (cfg->nq < 1 || cfg->nq > 65535)
+    return fail(SQC_HIP_ERR_BAD_ARG, "nq must be in 1..65535");
+  if (cfg->weight_kind < 0 || cfg->weight_kind > 2)
+    return fail(SQC_HIP_ERR_BAD_ARG, "unknown weight kind");
+  if (cfg->norm < 1 || cfg->norm > 3) return fail(SQC_HIP_ERR_BAD_ARG, "unknown norm");
+  if (cfg->weight_kind == SQC_HIP_WEIGHT_NEUTRON && !cfg->amp_const)
+    return fail(SQC_HIP_ERR_BAD_ARG, "neutron weights need amp_const");
+  if (cfg->weight_kind == SQC_HIP_WEIGHT_XRAY && !cfg->xray_coef)
+    return fail(SQC_HIP_ERR_BAD_ARG, "x-ray weights need xray_coef");
+  if (!(cfg->qmax > cfg->qmin) || cfg->qmin < 0.0)
+    return fail(SQC_HIP_ERR_BAD_ARG, "q range must be positive");
+  auto *h = new sqc_hip_plan();
+  const int rc = build_plan(h, cfg);
+  if (rc > SQC_HIP_WARN_EPS_TOO_SMALL) {
+    free_plan(h);
+    return rc;
+  }
+  *plan = h;
+  return rc;
+}
+
+int sqc_hip_plan_info(const sqc_hip_plan *plan, sqc_hip_info *info) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!info) return fail(SQC_HIP_ERR_BAD_ARG, "null info");
+  const sqc::Plan &p = plan->p;
+  std::memset(info, 0, sizeof(*info));
+  for (int i = 0; i < 3; ++i) {
+    info->modes[i] = p.m[i];
+    info->fine[i] = p.geo.n[i];
+  }
+  info->nmodes = p.nmodes;
+  info->nhalf = p.nhalf;
+  info->nspread = p.geo.ns;
+  info->nfields = p.geo.F;
+  info->beta = p.geo.beta;
+  info->upsampfac = p.sigma;
+  info->device_bytes = p.bytes;
+  info->nlines = p.geo.nlines;
+  info->tile[0] = p.geo.txr;
+  info->tile[1] = p.geo.ty;
+  info->tile[2] = p.tx2;
+  info->ncomp = p.geo.ncomp;
+  info->rho_len = p.ncand;
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_shells(const sqc_hip_plan *plan, double *den, int64_t *shell_count) {
+  if (int rc = check_plan(plan)) return rc;
+  const sqc::Plan &p = plan->p;
+  for (int s = 0; s < p.geo.nq; ++s) {
+    if (den) den[s] = p.den[s];
+    if (shell_count) shell_count[s] = p.shell_count[s];
+  }
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_modes(const sqc_hip_plan *plan, int32_t *hkl, double *qvec, int64_t *gidx) {
+  if (int rc = check_plan(plan)) return rc;
+  const sqc::Plan &p = plan->p;
+  ModeGeom mg{};
+  std::memcpy(mg.b, p.geo.b, sizeof(mg.b));
+  mg.qmin = p.cfg.qmin;
+  mg.qmax = p.cfg.qmax;
+  mg.nq = p.cfg.nq;
+  mg.dq = p.dq;
+  int64_t k = 0;
+  for (int64_t p3 = 1; p3 <= p.m[2]; ++p3) {
+    const int h3 = (int)(p3 - (p.m[2] + 1) / 2);
+    for (int64_t p2 = 1; p2 <= p.m[1]; ++p2) {
+      const int h2 = (int)(p2 - (p.m[1] + 1) / 2);
+      for (int64_t p1 = 1; p1 <= p.m[0]; ++p1) {
+        const int h1 = (int)(p1 - (p.m[0] + 1) / 2);
+        if (h1 == 0 && h2 == 0 && h3 == 0) continue;
+        double q[3];
+        const double ql = mg.qlen(h1, h2, h3, q);
+        if (!mg.kept(ql)) continue;
+        if (hkl) {
+          hkl[3 * k] = h1;
+          hkl[3 * k + 1] = h2;
+          hkl[3 * k + 2] = h3;
+        }
+        if (qvec) {
+          qvec[3 * k] = q[0];
+          qvec[3 * k + 1] = q[1];
+          qvec[3 * k + 2] = q[2];
+        }
+        if (gidx) gidx[k] = p1 + p.m[0] * (p2 - 1 + p.m[1] * (p3 - 1));
+        ++k;
+      }
+    }
+  }
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_accumulate_device(sqc_hip_plan *plan, const double *pos_device, int64_t nframes) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!pos_device || nframes < 0) return fail(SQC_HIP_ERR_BAD_ARG, "bad positions");
+  sqc::Plan &p = plan->p;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  const int64_t stride = 3 * p.geo.natoms;
+  for (int64_t f = 0; f < nframes; ++f) {
+    const int rc = sqc::launch_forward(p, pos_device + f * stride, 0, nullptr, nullptr);
+    if (rc) return hip_fail((hipError_t)rc, "forward launch");
+    ++p.nframes;
+  }
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_accumulate(sqc_hip_plan *plan, const double *pos, int64_t nframes) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!pos || nframes < 0) return fail(SQC_HIP_ERR_BAD_ARG, "bad positions");
+  sqc::Plan &p = plan->p;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  const int64_t stride = 3 * p.geo.natoms;
+  for (int64_t f0 = 0; f0 < nframes; f0 += p.batch_cap) {
+    const int64_t nb = std::min<int64_t>(p.batch_cap, nframes - f0);
+    HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+    HIP_TRY(hipMemcpyAsync(p.d_pos, pos + f0 * stride, sizeof(double) * stride * nb,
+                           hipMemcpyHostToDevice, p.stream),
+            "upload positions");
+    if (int rc = sqc_hip_accumulate_device(plan, p.d_pos, nb)) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_fetch(sqc_hip_plan *plan, double *num, double *partial, double *gnum,
+                  int64_t *nframes) {
+  if (int rc = check_plan(plan)) return rc;
+  sqc::Plan &p = plan->p;
+  const sqc::Geo &g = p.geo;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  const size_t ntot = (size_t)g.ncomp * g.nq;
+  std::vector<double> acc(2 * ntot);
+  HIP_TRY(hipMemcpy(acc.data(), p.d_acc, sizeof(double) * 2 * ntot, hipMemcpyDeviceToHost),
+          "download accumulators");
+  if (num)
+    for (int s = 0; s < g.nq; ++s) num[s] = acc[s] + acc[ntot + s];
+  if (partial) {
+    const int S = g.S;
+    std::fill(partial, partial + (size_t)S * S * g.nq, 0.0);
+    if (g.ncomp > 1) {
+      int pidx = 1;
+      for (int a = 0; a < S; ++a)
+        for (int b = a; b < S; ++b, ++pidx)
+          for (int s = 0; s < g.nq; ++s)
+            partial[((size_t)a * S + b) * g.nq + s] =
+                acc[(size_t)pidx * g.nq + s] + acc[ntot + (size_t)pidx * g.nq + s];
+    }
+  }
+  if (gnum) {
+    if (!p.d_gnum) return fail(SQC_HIP_ERR_BAD_ARG, "plan was created without want_grid");
+    std::vector<double> gc(p.ncand);
+    HIP_TRY(hipMemcpy(gc.data(), p.d_gnum, sizeof(double) * p.ncand, hipMemcpyDeviceToHost),
+            "download grid");
+    ModeGeom mg{};
+    std::memcpy(mg.b, g.b, sizeof(mg.b));
+    mg.qmin = p.cfg.qmin;
+    mg.qmax = p.cfg.qmax;
+    mg.nq = p.cfg.nq;
+    mg.dq = p.dq;
+    int64_t k = 0;
+    for (int64_t p3 = 1; p3 <= p.m[2]; ++p3) {
+      const int h3 = (int)(p3 - (p.m[2] + 1) / 2);
+      for (int64_t p2 = 1; p2 <= p.m[1]; ++p2) {
+        const int h2 = (int)(p2 - (p.m[1] + 1) / 2);
+        for (int64_t p1 = 1; p1 <= p.m[0]; ++p1) {
+          const int h1 = (int)(p1 - (p.m[0] + 1) / 2);
+          if (h1 == 0 && h2 == 0 && h3 == 0) continue;
+          if (!mg.kept(mg.qlen(h1, h2, h3))) continue;
+          int c1 = h1, c2 = h2, c3 = h3;
+          const bool canon = c3 > 0 || (c3 == 0 && (c2 > 0 || (c2 == 0 && c1 > 0)));
+          if (!canon) {
+            c1 = -c1;
+            c2 = -c2;
+            c3 = -c3;
+          }
+          const int line = p.loff[c3] + (c2 - p.lo2[c3]);
+          const int64_t c = p.line_off[line] + (c1 - p.line_info[4 * line + 2]);
+          gnum[k++] = gc[c];
+        }
+      }
+    }
+  }
+  if (nframes) *nframes = p.nframes;
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_reset(sqc_hip_plan *plan) {
+  if (int rc = check_plan(plan)) return rc;
+  sqc::Plan &p = plan->p;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  HIP_TRY(hipMemset(p.d_acc, 0, sizeof(double) * 2 * p.geo.ncomp * p.geo.nq), "hipMemset");
+  if (p.d_gnum) HIP_TRY(hipMemset(p.d_gnum, 0, sizeof(double) * p.ncand), "hipMemset");
+  p.nframes = 0;
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_forward_device(sqc_hip_plan *plan, const double *pos_device, int64_t nframes,
+                           double *out_device, void *rho_device, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!pos_device || !out_device || nframes < 0)
+    return fail(SQC_HIP_ERR_BAD_ARG, "bad forward arguments");
+  sqc::Plan &p = plan->p;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  StreamScope scope(p, stream);
+  const int64_t stride = 3 * p.geo.natoms;
+  const size_t vs = p.prec == 1 ? 8 : 16;
+  const size_t rstride = (size_t)p.geo.F * p.ncand * vs;
+  for (int64_t f = 0; f < nframes; ++f) {
+    void *rho = rho_device ? (void *)((char *)rho_device + f * rstride) : nullptr;
+    const int rc = sqc::launch_forward(p, pos_device + f * stride, 1,
+                                       out_device + f * p.geo.ncomp * p.geo.nq, rho);
+    if (rc) return hip_fail((hipError_t)rc, "forward launch");
+  }
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const void *rho_device,
+                           const double *gS_device, const double *gP_device,
+                           double *grad_device, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!pos_device || !rho_device || !gS_device || !grad_device)
+    return fail(SQC_HIP_ERR_BAD_ARG, "bad adjoint arguments");
+  sqc::Plan &p = plan->p;
+  if (gP_device && p.geo.ncomp == 1)
+    return fail(SQC_HIP_ERR_BAD_ARG, "pair gradients need a plan created with partials");
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  if (!p.d_slots) {
+    int rc = alloc(&p.d_slots, sizeof(double) * 3 * p.geo.natoms * p.geo.sx * p.geo.sy, p.bytes);
+    if (rc) return rc;
+  }
+  StreamScope scope(p, stream);
+  const int rc = sqc::launch_adjoint(p, pos_device, rho_device, gS_device, gP_device, grad_device);
+  if (rc) return hip_fail((hipError_t)rc, "adjoint launch");
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target,
+                     const double *wq, double *loss, double *grad) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!pos || !target || !grad) return fail(SQC_HIP_ERR_BAD_ARG, "bad gradient arguments");
+  sqc::Plan &p = plan->p;
+  const sqc::Geo &g = p.geo;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  const size_t vs = p.prec == 1 ? 8 : 16;
+  int rc;
+  if (!plan->g_pos) {
+    if ((rc = alloc(&plan->g_pos, sizeof(double) * 3 * g.natoms, p.bytes))) return rc;
+    if ((rc = alloc(&plan->g_out, sizeof(double) * g.ncomp * g.nq, p.bytes))) return rc;
+    if ((rc = alloc(&plan->g_gs, sizeof(double) * g.nq, p.bytes))) return rc;
+    if ((rc = alloc(&plan->g_grad, sizeof(double) * 3 * g.natoms, p.bytes))) return rc;
+    if ((rc = alloc((char **)&plan->g_rho, g.F * p.ncand * vs, p.bytes))) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(plan->g_pos, pos, sizeof(double) * 3 * g.natoms, hipMemcpyHostToDevice,
+                         p.stream),
+          "upload positions");
+  if ((rc = sqc_hip_forward_device(plan, plan->g_pos, 1, plan->g_out, plan->g_rho, nullptr)))
+    return rc;
+  std::vector<double> num(g.nq), gs(g.nq);
+  HIP_TRY(hipMemcpyAsync(num.data(), plan->g_out, sizeof(double) * g.nq, hipMemcpyDeviceToHost,
+                         p.stream),
+          "download shells");
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  double L = 0.0;
+  for (int s = 0; s < g.nq; ++s) {
+    const double S = p.den[s] > 0.0 ? num[s] / p.den[s] : 0.0;
+    const double w = wq ? wq[s] : 1.0;
+    const double r = S - target[s];
+    L += w * r * r;
+    gs[s] = 2.0 * w * r;
+  }
+  if (loss) *loss = L;
+  HIP_TRY(hipMemcpyAsync(plan->g_gs, gs.data(), sizeof(double) * g.nq, hipMemcpyHostToDevice,
+                         p.stream),
+          "upload gS");
+  if ((rc = sqc_hip_adjoint_device(plan, plan->g_pos, plan->g_rho, plan->g_gs, nullptr,
+                                   plan->g_grad, nullptr)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(grad, plan->g_grad, sizeof(double) * 3 * g.natoms,
+                         hipMemcpyDeviceToHost, p.stream),
+          "download gradient");
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_destroy(sqc_hip_plan *plan) {
+  if (!plan) return SQC_HIP_OK;
+  (void)hipSetDevice(plan->p.device);
+  (void)hipStreamSynchronize(plan->p.stream);
+  free_plan(plan);
+  return SQC_HIP_OK;
+}
+
+const char *sqc_hip_error_string(int code) {
+  switch (code) {
+    case SQC_HIP_OK: return "success";
+    case SQC_HIP_WARN_EPS_TOO_SMALL: return "requested tolerance too small; clipped";
+    case SQC_HIP_ERR_MAXNALLOC: return "grid too large";
+    case SQC_HIP_ERR_ALLOC: return "allocation failed";
+    case SQC_HIP_ERR_BAD_ARG: return "invalid argument";
+    case SQC_HIP_ERR_ABI: return "sqc_hip_config ABI mismatch";
+    case SQC_HIP_ERR_NO_DEVICE: return "no usable HIP device";
+    case SQC_HIP_ERR_HIP: return "HIP runtime error";
+    case SQC_HIP_ERR_NO_MODES: return "no reciprocal lattice points in the q range";
+    default: return "unknown error";
+  }
+}
+
+const char *sqc_hip_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// sqc_internal.hpp -- plan layout shared by the host code and the gfx950 kernels.
+//
+// Data layout in HBM (F = fields, i.e. species transforms or one weighted
+// transform; T2 = float2 or double2; n = fine grid, M = (m-1)/2):
+//   pos      [batch][natoms][3]       f64   Cartesian input, frame_t%pos order
+//   X        [natoms][3]              f64   grid coordinates s*n of the frame
+//   W1       [F][M3+1][n2][n1]        T2    after spread + z-DFT, h3 >= 0 only
+//   W2       [F][nlines][n1]          T2    after the y-DFT, canonical (h2,h3) lines
+//   stab     [ncand]                  u16   shell of each candidate mode (0xFFFF: not kept)
+//   wgpart   [nblk][ncomp][nq]        f64   per-workgroup shell partials of one frame
+//   acc      [2][ncomp][nq]           f64   run accumulators (sum, compensation)
+// Real strengths make F[g] real, so only the half spectrum h3 > 0, or h3 = 0
+// and h2 > 0, or h3 = h2 = 0 and h1 > 0 is transformed; its modes count twice.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sqc_hip.h"
+
+namespace sqc {
+
+constexpr int kMaxStages = 12;
+constexpr int kMaxSpecies = 8;
+constexpr uint16_t kNoShell = 0xFFFF;
+
+// Mixed-radix factorisation of one axis.
+struct AxisPlan {
+  int n;
+  int nst;
+  int rad[kMaxStages];
+};
+
+// Everything a kernel needs to know about the geometry, passed by value.
+struct Geo {
+  double ainv[9];   // A^-1, column-major
+  double origin[3];
+  double b[9];      // reciprocal basis, columns b1 b2 b3
+  int n[3];         // fine grid
+  int M[3];         // (m - 1) / 2
+  int ns;
+  double beta;
+  double half_ns;
+  // spread tiles (real x-columns x rows) and the bins that sort atoms by tile
+  int txr, ty, ntx, nty, nbins;
+  int hx, hy;       // bin halo each side
+  int sx, sy;       // adjoint gradient slots per atom per axis
+  int F;            // fields
+  int S;            // species
+  int ncomp;        // 1 + number of species pairs (when partials)
+  int nq;
+  int weight_kind;
+  int nlines;
+  int partials;
+  int64_t natoms;
+  double amp[kMaxSpecies];
+  double xcoef[kMaxSpecies][9];
+};
+
+struct Plan {
+  sqc_hip_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int prec = 0;               // 0 f64, 1 f32
+  Geo geo{};
+  int64_t m[3]{};
+  double sigma = 2.0;
+  double dq = 0.0;
+  int64_t nmodes = 0;         // K
+  int64_t nhalf = 0;          // canonical kept modes
+  int64_t ncand = 0;          // candidate slots in the line table
+  std::vector<int32_t> species_of;
+  std::vector<int64_t> counts;  // atoms per species
+  std::vector<double> den;
+  std::vector<int64_t> shell_count;
+  // per h3 = 0..M3: first line and the h2 interval
+  std::vector<int32_t> lo2, hi2, loff;
+  // per line: (h2, h3, lo1, hi1) and the candidate offset
+  std::vector<int32_t> line_info;  // 4 per line
+  std::vector<int32_t> line_off;
+  AxisPlan ax[3]{};
+  // launch shapes
+  int tx2 = 16;               // y-pass x-columns per workgroup
+  int lb = 8;                 // x-pass lines per workgroup step
+  int nblk3 = 0;              // x-pass persistent workgroups
+  size_t lds1 = 0, lds2 = 0, lds3 = 0;
+  int64_t nframes = 0;
+  int64_t batch_cap = 1;
+  // device buffers
+  double *d_pos = nullptr;
+  double *d_X = nullptr;
+  int32_t *d_key = nullptr, *d_count = nullptr, *d_start = nullptr, *d_cursor = nullptr,
+          *d_sidx = nullptr, *d_species = nullptr;
+  void *d_W1 = nullptr, *d_W2 = nullptr, *d_tw[3] = {nullptr, nullptr, nullptr};
+  double *d_ph[3] = {nullptr, nullptr, nullptr};
+  int32_t *d_lo2 = nullptr, *d_hi2 = nullptr, *d_loff = nullptr, *d_line_info = nullptr,
+          *d_line_off = nullptr;
+  uint16_t *d_stab = nullptr;
+  double *d_den = nullptr, *d_cnt = nullptr;
+  double *d_wgpart = nullptr, *d_acc = nullptr, *d_frame = nullptr;
+  double *d_gnum = nullptr;
+  double *d_slots = nullptr;
+  double *d_gs = nullptr;
+  void *d_rho = nullptr;
+  int64_t bytes = 0;
+};
+
+// Launchers (sqc_kernels.hip).  All run on p.stream.
+int launch_forward(Plan &p, const double *d_pos_frame, int mode, double *frame_out,
+                   void *rho_out);
+int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const double *gS,
+                   const double *gP, double *grad);
+int launch_finalize(Plan &p, double *frame_out, int accumulate);
+void kernel_lds_limits(Plan &p);
+
+}  // namespace sqc

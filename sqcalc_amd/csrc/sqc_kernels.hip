@@ -1,0 +1,1037 @@
+// sqc_kernels.hip -- gfx950 kernels of the reciprocal-space S(q) path.
+//
+// Forward, per frame (one launch each):
+//   k_prep        wrap the atoms into the cell (FP64), grid coordinates, tile bin
+//   k_scan        exclusive scan of the bin counts (one workgroup)
+//   k_scatter     counting-sort scatter of atom indices by bin
+//   k_binsort     order each bin by atom index (makes the spread deterministic)
+//   k_spread_z    per (x,y) tile: spread into LDS, z-DFT in LDS, write h3 >= 0
+//   k_fft_y       per (h3, x-chunk): y-DFT in LDS, write the canonical lines
+//   k_fft_x_bin   per group of lines: x-DFT in LDS, deconvolve, combine species,
+//                 shell sums in LDS, per-workgroup partials
+//   k_finalize    ordered, compensated sum of the workgroup partials
+// Adjoint (type-2 with the kernel derivative): k_adj_x, k_adj_y,
+// k_adj_z_gather, k_adj_sum.
+//
+// All DFTs carry the +i sign of the reference transform
+// (src/sqc_structure_factor.f90:1069-1071, iflag = +1) and are mixed-radix
+// Stockham passes over lines held in LDS, with the radix-R butterflies of a
+// stage staged through registers so that the pass runs in place.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "sqc_internal.hpp"
+
+namespace sqc {
+
+constexpr int NT = 256;  // threads per workgroup for every LDS kernel
+
+template <typename T> struct C2;
+template <> struct C2<float> { using t = float2; };
+template <> struct C2<double> { using t = double2; };
+
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename V> __device__ __forceinline__ V cmul_i(V a) { return V{-a.y, a.x}; }  // i a
+template <typename V, typename S> __device__ __forceinline__ V cscale(V a, S s) {
+  return V{a.x * s, a.y * s};
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs, y_k = sum_r v_r exp(+2 pi i r k / R)
+// ---------------------------------------------------------------------------
+template <typename T, int R> struct Dft;
+
+template <typename T> struct Dft<T, 2> {
+  using V = typename C2<T>::t;
+  static __device__ __forceinline__ void run(V *v) {
+    V a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <typename T> struct Dft<T, 3> {
+  using V = typename C2<T>::t;
+  static __device__ __forceinline__ void run(V *v) {
+    const T c = T(-0.5);
+    const T s = T(0.86602540378443864676372317075294);
+    V t1 = cadd(v[1], v[2]);
+    V d = cscale(cmul_i(csub(v[1], v[2])), s);
+    V m = cadd(v[0], cscale(t1, c));
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, d);
+    v[2] = csub(m, d);
+  }
+};
+
+template <typename T> struct Dft<T, 4> {
+  using V = typename C2<T>::t;
+  static __device__ __forceinline__ void run(V *v) {
+    V t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    V t2 = cadd(v[1], v[3]), t3 = cmul_i(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <typename T> struct Dft<T, 5> {
+  using V = typename C2<T>::t;
+  static __device__ __forceinline__ void run(V *v) {
+    const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
+    const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
+    const T s1 = T(0.95105651629515357211643933337938);    // sin(2pi/5)
+    const T s2 = T(0.58778525229247312916870595463907);    // sin(4pi/5)
+    V b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+    V d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    V t1 = cadd(v[0], cadd(cscale(b1, c1), cscale(b2, c2)));
+    V t2 = cadd(v[0], cadd(cscale(b1, c2), cscale(b2, c1)));
+    V u1 = cmul_i(cadd(cscale(d1, s1), cscale(d2, s2)));
+    V u2 = cmul_i(csub(cscale(d1, s2), cscale(d2, s1)));
+    v[0] = cadd(v[0], cadd(b1, b2));
+    v[1] = cadd(t1, u1);
+    v[4] = csub(t1, u1);
+    v[2] = cadd(t2, u2);
+    v[3] = csub(t2, u2);
+  }
+};
+
+template <typename T> struct Dft<T, 8> {
+  using V = typename C2<T>::t;
+  static __device__ __forceinline__ void run(V *v) {
+    const T r = T(0.70710678118654752440084436210485);
+    V e[4] = {v[0], v[2], v[4], v[6]};
+    V o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<T, 4>::run(e);
+    Dft<T, 4>::run(o);
+    // w8^k o_k, w8 = exp(i pi / 4)
+    V w1 = V{(o[1].x - o[1].y) * r, (o[1].x + o[1].y) * r};
+    V w2 = cmul_i(o[2]);
+    V w3 = V{(-o[3].x - o[3].y) * r, (o[3].x - o[3].y) * r};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
+  }
+};
+
+// Max LDS elements a workgroup transforms at once: each thread stages at
+// most ceil(MAXE / (R NT)) radix-R butterflies in registers.
+template <typename T> struct FftCap;
+template <> struct FftCap<float> { static constexpr int maxe = 8192; };
+template <> struct FftCap<double> { static constexpr int maxe = 4096; };
+
+// One Stockham stage over nlines lines of length n.  Element (line, pos) is
+// lds[line * ls + pos * ps].  line_fast selects the thread order: lines
+// fastest (interleaved layout, ls = 1) or butterflies fastest (ps = 1).
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(typename C2<T>::t *lds, int nlines, int n, int Ns, int ls,
+                                          int ps, const typename C2<T>::t *tw, bool line_fast) {
+  using V = typename C2<T>::t;
+  constexpr int MAXB = (FftCap<T>::maxe + R * NT - 1) / (R * NT);
+  const int nb = n / R;
+  const int total = nlines * nb;
+  const int tstep = n / (Ns * R);
+  const int tid = threadIdx.x;
+  V v[MAXB][R];
+  int base[MAXB], jj[MAXB];
+#pragma unroll
+  for (int it = 0; it < MAXB; ++it) {
+    const int b = tid + it * NT;
+    jj[it] = -1;
+    if (b < total) {
+      int line, j;
+      if (line_fast) {
+        line = b % nlines;
+        j = b / nlines;
+      } else {
+        j = b % nb;
+        line = b / nb;
+      }
+      jj[it] = j;
+      base[it] = line * ls;
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        V a = lds[base[it] + (j + r * nb) * ps];
+        if (r > 0 && k > 0) a = cmul(a, tw[k * r * tstep]);
+        v[it][r] = a;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXB; ++it) {
+    if (jj[it] >= 0) {
+      Dft<T, R>::run(v[it]);
+      const int j = jj[it];
+      const int k = j % Ns;
+      const int d0 = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[base[it] + (d0 + r * Ns) * ps] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void fft_lines(typename C2<T>::t *lds, int nlines, const AxisPlan &ax, int ls, int ps,
+                          const typename C2<T>::t *tw, bool line_fast) {
+  int Ns = 1;
+  for (int s = 0; s < ax.nst; ++s) {
+    const int R = ax.rad[s];
+    switch (R) {
+      case 2: fft_stage<T, 2>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
+      case 3: fft_stage<T, 3>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
+      case 4: fft_stage<T, 4>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
+      case 5: fft_stage<T, 5>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
+      default: fft_stage<T, 8>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
+    }
+    Ns *= R;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ES kernel phi(u) = exp(beta (sqrt(1 - z^2) - 1)), z = 2 u / ns, and dphi/du.
+// The argument is formed in FP64 (u comes from FP64 grid coordinates); the
+// exponential runs in the path precision.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T es_exp(double a);
+template <> __device__ __forceinline__ float es_exp<float>(double a) { return __expf((float)a); }
+template <> __device__ __forceinline__ double es_exp<double>(double a) { return exp(a); }
+
+template <typename T>
+__device__ __forceinline__ T es_phi(double u, double ns, double beta, T *dphi) {
+  const double z = 2.0 * u / ns;
+  const double q = 1.0 - z * z;
+  if (q <= 0.0) {
+    if (dphi) *dphi = T(0);
+    return T(0);
+  }
+  const double sq = sqrt(q);
+  const T ph = es_exp<T>(beta * (sq - 1.0));
+  if (dphi) *dphi = (T)((double)ph * (-beta * z / sq) * (2.0 / ns));
+  return ph;
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// Overlap of the stencil [i0, i0 + ns) (periodic in n) with [t0, t0 + tw):
+// returns the count and sets the first stencil offset and the first local index.
+__device__ __forceinline__ int overlap(int i0, int ns, int n, int t0, int tw, int *k_lo, int *l_lo) {
+#pragma unroll
+  for (int sh = -1; sh <= 1; ++sh) {
+    const int a = i0 + sh * n;
+    const int lo = max(a, t0), hi = min(a + ns, t0 + tw);
+    if (lo < hi) {
+      *k_lo = lo - a;
+      *l_lo = lo - t0;
+      return hi - lo;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Atom preparation and counting sort
+// ---------------------------------------------------------------------------
+__global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__restrict__ species,
+                       double *__restrict__ X, int32_t *__restrict__ key,
+                       int32_t *__restrict__ count) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.natoms) return;
+  const double d0 = pos[3 * j] - g.origin[0];
+  const double d1 = pos[3 * j + 1] - g.origin[1];
+  const double d2 = pos[3 * j + 2] - g.origin[2];
+  double x[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double s = g.ainv[i] * d0 + g.ainv[i + 3] * d1 + g.ainv[i + 6] * d2;
+    s = s - floor(s);
+    double u = s * (double)g.n[i];
+    if (u >= (double)g.n[i]) u -= (double)g.n[i];
+    x[i] = u;
+    X[3 * j + i] = u;
+  }
+  const int tx = min((int)x[0] / g.txr, g.ntx - 1);
+  const int ty = min((int)x[1] / g.ty, g.nty - 1);
+  const int f = (g.F == 1) ? 0 : species[j];
+  const int k = (f * g.nty + ty) * g.ntx + tx;
+  key[j] = k;
+  atomicAdd(&count[k], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_scan(int32_t *__restrict__ count, int32_t *__restrict__ start,
+                                               int32_t *__restrict__ cursor, int nb) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int chunk = (nb + 1023) / 1024;
+  const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
+  int s = 0;
+  for (int b = b0; b < b1; ++b) s += count[b];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = tid ? part[tid - 1] : 0;
+  for (int b = b0; b < b1; ++b) {
+    start[b] = run;
+    cursor[b] = run;
+    run += count[b];
+    count[b] = 0;  // ready for the next frame
+  }
+  if (tid == 1023) start[nb] = part[1023];
+}
+
+__global__ void k_scatter(int64_t natoms, const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
+                          int32_t *__restrict__ sidx) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= natoms) return;
+  const int p = atomicAdd(&cursor[key[j]], 1);
+  sidx[p] = (int32_t)j;
+}
+
+__global__ void k_binsort(int nb, const int32_t *__restrict__ start, int32_t *__restrict__ sidx) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int k0 = start[b], k1 = start[b + 1];
+  for (int k = k0 + 1; k < k1; ++k) {
+    const int v = sidx[k];
+    int i = k - 1;
+    while (i >= k0 && sidx[i] > v) {
+      sidx[i + 1] = sidx[i];
+      --i;
+    }
+    sidx[i + 1] = v;
+  }
+}
+
+// Bins to visit around a tile, without visiting a bin twice on small grids.
+__device__ __forceinline__ void bin_window(int t, int h, int nt, int *first, int *count) {
+  if (2 * h + 1 >= nt) {
+    *first = 0;
+    *count = nt;
+  } else {
+    *first = t - h;
+    *count = 2 * h + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spread + z-DFT.  One workgroup per (x,y) tile of txr x ty real columns and
+// one field.  The tile is held in LDS as txr/2 x ty complex columns of n3
+// points: real column 2p in .x, 2p+1 in .y, so one complex DFT transforms two
+// real columns.  Wave w owns the z slab [w n3/4, (w+1) n3/4) and walks the
+// candidate atoms in sorted order, so every LDS cell is updated by one wave,
+// in atom order: no atomics, and the grid is bit-reproducible.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const double *__restrict__ X,
+                                                 const int32_t *__restrict__ sidx,
+                                                 const int32_t *__restrict__ start,
+                                                 const int32_t *__restrict__ species,
+                                                 const typename C2<T>::t *__restrict__ twz,
+                                                 typename C2<T>::t *__restrict__ W1) {
+  using V = typename C2<T>::t;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+  const int ccx = g.txr / 2, cc = ccx * g.ty;
+  V *tile = reinterpret_cast<V *>(smem);
+  V *tw = tile + (size_t)n3 * cc;
+  T *tflat = reinterpret_cast<T *>(tile);
+  const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx, f = blockIdx.y;
+  const int x0 = tx * g.txr, y0 = ty * g.ty;
+  const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n3 * cc; i += NT) tile[i] = V{T(0), T(0)};
+  for (int i = tid; i < n3; i += NT) tw[i] = twz[i];
+  __syncthreads();
+
+  const int wave = tid >> 6, lane = tid & 63, nw = NT / 64;
+  const int zs0 = (n3 * wave) / nw, zs1 = (n3 * (wave + 1)) / nw;
+  const int ns = g.ns;
+  int bxs, bxn, bys, byn;
+  bin_window(tx, g.hx, g.ntx, &bxs, &bxn);
+  bin_window(ty, g.hy, g.nty, &bys, &byn);
+  for (int iby = 0; iby < byn; ++iby) {
+    const int by = wrapi(bys + iby, g.nty);
+    for (int ibx = 0; ibx < bxn; ++ibx) {
+      const int bx = wrapi(bxs + ibx, g.ntx);
+      const int bin = (f * g.nty + by) * g.ntx + bx;
+      const int k0 = start[bin], k1 = start[bin + 1];
+      for (int k = k0; k < k1; ++k) {
+        const int j = sidx[k];
+        const double Xx = X[3 * j], Xy = X[3 * j + 1], Xz = X[3 * j + 2];
+        const int i0x = (int)ceil(Xx - g.half_ns);
+        const int i0y = (int)ceil(Xy - g.half_ns);
+        const int i0z = (int)ceil(Xz - g.half_ns);
+        int kxl, xll, kyl, yll;
+        const int ox = overlap(i0x, ns, n1, x0, txw, &kxl, &xll);
+        if (ox == 0) continue;
+        const int oy = overlap(i0y, ns, n2, y0, tyw, &kyl, &yll);
+        if (oy == 0) continue;
+        // z pieces of the stencil inside this wave's slab
+        const int a = wrapi(i0z, n3);
+        int pk[2], pz[2], pc[2], np = 0;
+        {
+          const int e1 = min(a + ns, n3);
+          const int lo = max(a, zs0), hi = min(e1, zs1);
+          if (lo < hi) { pk[np] = lo - a; pz[np] = lo; pc[np] = hi - lo; ++np; }
+          if (a + ns > n3) {
+            const int e2 = a + ns - n3;
+            const int lo2 = max(0, zs0), hi2 = min(e2, zs1);
+            if (lo2 < hi2) { pk[np] = n3 - a + lo2; pz[np] = lo2; pc[np] = hi2 - lo2; ++np; }
+          }
+        }
+        if (np == 0) continue;
+        const T w = (g.F == 1) ? (T)g.amp[species[j]] : T(1);
+        T phi = T(0);
+        if (lane < 3 * ns) {
+          const int ax = lane / ns, kk = lane - ax * ns;
+          const double X3 = ax == 0 ? Xx : (ax == 1 ? Xy : Xz);
+          const int i03 = ax == 0 ? i0x : (ax == 1 ? i0y : i0z);
+          phi = es_phi<T>((double)(i03 + kk) - X3, (double)ns, g.beta, (T *)nullptr);
+        }
+        for (int p = 0; p < np; ++p) {
+          const int cells = ox * oy * pc[p];
+          const int iters = (cells + 63) / 64;
+          for (int it = 0; it < iters; ++it) {
+            // every lane takes part in the shuffles; only c < cells writes
+            const int c = lane + 64 * it;
+            const int cv = c < cells ? c : 0;
+            const int ix = cv % ox;
+            const int t2 = cv / ox;
+            const int iy = t2 % oy;
+            const int kz = t2 / oy;
+            const T fx = __shfl(phi, kxl + ix);
+            const T fy = __shfl(phi, ns + kyl + iy);
+            const T fz = __shfl(phi, 2 * ns + pk[p] + kz);
+            if (c < cells) {
+              const int xl = xll + ix, yl = yll + iy, z = pz[p] + kz;
+              const int col = yl * ccx + (xl >> 1);
+              T *cell = tflat + 2 * ((size_t)z * cc + col) + (xl & 1);
+              *cell += w * fx * fy * fz;
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  fft_lines<T>(tile, cc, az, 1, cc, tw, true);
+  // split the packed pairs and write h3 = 0..M3
+  const int M3 = g.M[2];
+  const int per = tyw * txw;
+  for (int idx = tid; idx < (M3 + 1) * per; idx += NT) {
+    const int xl = idx % txw;
+    const int r = idx / txw;
+    const int yl = r % tyw;
+    const int k3 = r / tyw;
+    const int col = yl * ccx + (xl >> 1);
+    const V Z = tile[(size_t)k3 * cc + col];
+    const V Zm = tile[(size_t)((n3 - k3) % n3) * cc + col];
+    V o;
+    if ((xl & 1) == 0)
+      o = V{T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y - Zm.y)};
+    else
+      o = V{T(0.5) * (Z.y + Zm.y), T(-0.5) * (Z.x - Zm.x)};
+    W1[(((size_t)f * (M3 + 1) + k3) * n2 + (y0 + yl)) * n1 + x0 + xl] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y-DFT.  One workgroup per (x-chunk, h3, field).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
+                                              const typename C2<T>::t *__restrict__ W1,
+                                              const typename C2<T>::t *__restrict__ twy,
+                                              const int32_t *__restrict__ lo2,
+                                              const int32_t *__restrict__ hi2,
+                                              const int32_t *__restrict__ loff,
+                                              typename C2<T>::t *__restrict__ W2) {
+  using V = typename C2<T>::t;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2];
+  V *cols = reinterpret_cast<V *>(smem);
+  V *tw = cols + (size_t)n2 * tx2;
+  const int x0 = blockIdx.x * tx2, k3 = blockIdx.y, f = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int lo = lo2[k3], hi = hi2[k3];
+  if (hi < lo) return;
+  for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
+  const V *src = W1 + ((size_t)f * (M3 + 1) + k3) * n2 * n1;
+  for (int idx = tid; idx < n2 * tx2; idx += NT) {
+    const int xl = idx % tx2, y = idx / tx2;
+    const int x = x0 + xl;
+    cols[idx] = x < n1 ? src[(size_t)y * n1 + x] : V{T(0), T(0)};
+  }
+  __syncthreads();
+  fft_lines<T>(cols, tx2, ay, 1, tx2, tw, true);
+  const int nl = hi - lo + 1;
+  V *dst = W2 + ((size_t)f * g.nlines + loff[k3]) * n1;
+  for (int idx = tid; idx < nl * tx2; idx += NT) {
+    const int xl = idx % tx2, r = idx / tx2;
+    const int x = x0 + xl;
+    if (x >= n1) continue;
+    const int h2 = lo + r;
+    const int yi = h2 < 0 ? h2 + n2 : h2;
+    dst[(size_t)r * n1 + x] = cols[yi * tx2 + xl];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x-DFT, deconvolution and shell sums.  Persistent workgroups walk groups of
+// lb lines; every field of a line is transformed together so that the pair
+// products of one mode are formed in registers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double xray_f(const double *c, double q) {
+  const double st = q * (1.0 / (4.0 * 3.14159265358979323846));
+  const double st2 = st * st;
+  double f = c[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f += c[i] * exp(-c[4 + i] * st2);
+  return f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, int ngroups,
+                                                  const typename C2<T>::t *__restrict__ W2,
+                                                  const typename C2<T>::t *__restrict__ twx,
+                                                  const int4 *__restrict__ line_info,
+                                                  const int32_t *__restrict__ line_off,
+                                                  const uint16_t *__restrict__ stab,
+                                                  const double *__restrict__ ph1,
+                                                  const double *__restrict__ ph2,
+                                                  const double *__restrict__ ph3,
+                                                  double *__restrict__ wgpart,
+                                                  typename C2<T>::t *__restrict__ rho_out,
+                                                  int64_t ncand, double *__restrict__ gnum) {
+  using V = typename C2<T>::t;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = g.n[0], F = g.F, nq = g.nq, ncomp = g.ncomp;
+  V *rows = reinterpret_cast<V *>(smem);
+  V *tw = rows + (size_t)lb * F * n1;
+  double *acc = reinterpret_cast<double *>(tw + n1);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
+  for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
+  const int S = g.S;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int l0 = grp * lb;
+    const int nl = min(lb, g.nlines - l0);
+    __syncthreads();
+    for (int idx = tid; idx < nl * F * n1; idx += NT) {
+      const int x = idx % n1, r = idx / n1;
+      const int li = r / F, f = r % F;
+      rows[idx] = W2[((size_t)f * g.nlines + l0 + li) * n1 + x];
+    }
+    __syncthreads();
+    fft_lines<T>(rows, nl * F, ax1, n1, 1, tw, false);
+    const int c0 = line_off[l0], c1 = line_off[l0 + nl];
+    for (int c = c0 + tid; c < c1; c += NT) {
+      const unsigned s = stab[c];
+      if (s == kNoShell) continue;
+      int li = 0;
+      while (li + 1 < nl && line_off[l0 + li + 1] <= c) ++li;
+      const int4 info = line_info[l0 + li];
+      const int h2 = info.x, h3 = info.y, lo1 = info.z;
+      const int h1 = lo1 + (c - line_off[l0 + li]);
+      const double d = 1.0 / (ph1[abs(h1)] * ph2[abs(h2)] * ph3[h3]);
+      const int ix = h1 < 0 ? h1 + n1 : h1;
+      double rr[kMaxSpecies], ri[kMaxSpecies];
+      for (int f = 0; f < F; ++f) {
+        const V v = rows[((size_t)li * F + f) * n1 + ix];
+        rr[f] = (double)v.x * d;
+        ri[f] = (double)v.y * d;
+        if (rho_out) rho_out[(size_t)f * ncand + c] = V{(T)rr[f], (T)ri[f]};
+      }
+      double tr, ti;
+      if (F == 1) {
+        tr = rr[0];
+        ti = ri[0];
+      } else {
+        double q = 0.0;
+        if (g.weight_kind == 2) {
+          const double h1d = h1, h2d = h2, h3d = h3;
+          const double qx = g.b[0] * h1d + g.b[3] * h2d + g.b[6] * h3d;
+          const double qy = g.b[1] * h1d + g.b[4] * h2d + g.b[7] * h3d;
+          const double qz = g.b[2] * h1d + g.b[5] * h2d + g.b[8] * h3d;
+          q = sqrt(qx * qx + qy * qy + qz * qz);
+        }
+        tr = 0.0;
+        ti = 0.0;
+        for (int a = 0; a < S; ++a) {
+          const double w = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
+          tr += w * rr[a];
+          ti += w * ri[a];
+        }
+      }
+      const double val = tr * tr + ti * ti;
+      atomicAdd(&acc[s], 2.0 * val);
+      if (gnum) gnum[c] += val;
+      if (g.partials && F > 1) {
+        int pidx = 1;
+        for (int a = 0; a < S; ++a)
+          for (int b = a; b < S; ++b, ++pidx)
+            atomicAdd(&acc[pidx * nq + s], 2.0 * (rr[a] * rr[b] + ri[a] * ri[b]));
+      }
+    }
+  }
+  __syncthreads();
+  double *out = wgpart + (size_t)blockIdx.x * ncomp * nq;
+  for (int i = tid; i < ncomp * nq; i += NT) out[i] = acc[i];
+}
+
+// Ordered, compensated (Neumaier) sum over the workgroup partials.
+__global__ void k_finalize(const double *__restrict__ wgpart, int nblk, int ntot,
+                           double *__restrict__ acc, double *__restrict__ frame_out, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  double s = 0.0, c = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    const double v = wgpart[(size_t)b * ntot + i];
+    const double t = s + v;
+    c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  const double total = s + c;
+  if (frame_out) frame_out[i] = total;
+  if (accumulate) {
+    const double a = acc[i];
+    double cc = acc[ntot + i];
+    const double t = a + total;
+    cc += (fabs(a) >= fabs(total)) ? (a - t) + total : (total - t) + a;
+    acc[i] = t;
+    acc[ntot + i] = cc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint.  V_c(x) = sum_m C_cm exp(i h_m . x) is real; its gradient at the
+// atoms is interpolated with the kernel derivative.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int ngroups, int Fa,
+                                              const typename C2<T>::t *__restrict__ rho,
+                                              int64_t ncand,
+                                              const typename C2<T>::t *__restrict__ twx,
+                                              const int4 *__restrict__ line_info,
+                                              const int32_t *__restrict__ line_off,
+                                              const uint16_t *__restrict__ stab,
+                                              const double *__restrict__ ph1,
+                                              const double *__restrict__ ph2,
+                                              const double *__restrict__ ph3,
+                                              const double *__restrict__ den,
+                                              const double *__restrict__ cnt,
+                                              const double *__restrict__ gS,
+                                              const double *__restrict__ gP,
+                                              typename C2<T>::t *__restrict__ W2) {
+  using V = typename C2<T>::t;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = g.n[0], F = g.F, nq = g.nq, S = g.S;
+  V *rows = reinterpret_cast<V *>(smem);
+  V *tw = rows + (size_t)lb * Fa * n1;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int l0 = grp * lb;
+    const int nl = min(lb, g.nlines - l0);
+    __syncthreads();
+    for (int idx = tid; idx < nl * Fa * n1; idx += NT) rows[idx] = V{T(0), T(0)};
+    __syncthreads();
+    const int c0 = line_off[l0], c1 = line_off[l0 + nl];
+    for (int c = c0 + tid; c < c1; c += NT) {
+      const unsigned s = stab[c];
+      if (s == kNoShell) continue;
+      int li = 0;
+      while (li + 1 < nl && line_off[l0 + li + 1] <= c) ++li;
+      const int4 info = line_info[l0 + li];
+      const int h2 = info.x, h3 = info.y, lo1 = info.z;
+      const int h1 = lo1 + (c - line_off[l0 + li]);
+      const double d = 1.0 / (ph1[abs(h1)] * ph2[abs(h2)] * ph3[h3]);
+      const int ix = h1 < 0 ? h1 + n1 : h1;
+      double rr[kMaxSpecies], ri[kMaxSpecies], w[kMaxSpecies];
+      for (int f = 0; f < F; ++f) {
+        const V v = rho[(size_t)f * ncand + c];
+        rr[f] = v.x;
+        ri[f] = v.y;
+      }
+      double q = 0.0;
+      if (g.weight_kind == 2) {
+        const double h1d = h1, h2d = h2, h3d = h3;
+        const double qx = g.b[0] * h1d + g.b[3] * h2d + g.b[6] * h3d;
+        const double qy = g.b[1] * h1d + g.b[4] * h2d + g.b[7] * h3d;
+        const double qz = g.b[2] * h1d + g.b[5] * h2d + g.b[8] * h3d;
+        q = sqrt(qx * qx + qy * qy + qz * qz);
+      }
+      double tr = 0.0, ti = 0.0;
+      if (F == 1) {
+        tr = rr[0];
+        ti = ri[0];
+      } else {
+        for (int a = 0; a < S; ++a) {
+          w[a] = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
+          tr += w[a] * rr[a];
+          ti += w[a] * ri[a];
+        }
+      }
+      const double G = den[s] > 0.0 ? gS[s] / den[s] : 0.0;
+      for (int cf = 0; cf < Fa; ++cf) {
+        const double wc = (Fa == 1) ? 1.0 : w[cf];
+        double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
+        if (gP) {
+          const double nc = (double)g.natoms * fmax(cnt[s], 1.0);
+          for (int a = 0; a < S; ++a)
+            for (int b = a; b < S; ++b) {
+              const double H = gP[((size_t)a * S + b) * nq + s] / nc;
+              if (cf == a) { cr += H * rr[b]; ci -= H * ri[b]; }
+              if (cf == b) { cr += H * rr[a]; ci -= H * ri[a]; }
+            }
+        }
+        rows[((size_t)li * Fa + cf) * n1 + ix] = V{(T)(cr * d), (T)(ci * d)};
+      }
+    }
+    __syncthreads();
+    fft_lines<T>(rows, nl * Fa, ax1, n1, 1, tw, false);
+    for (int idx = tid; idx < nl * Fa * n1; idx += NT) {
+      const int x = idx % n1, r = idx / n1;
+      const int li = r / Fa, cf = r % Fa;
+      W2[((size_t)cf * g.nlines + l0 + li) * n1 + x] = rows[idx];
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
+                                              const typename C2<T>::t *__restrict__ W2,
+                                              const typename C2<T>::t *__restrict__ twy,
+                                              const int32_t *__restrict__ lo2,
+                                              const int32_t *__restrict__ hi2,
+                                              const int32_t *__restrict__ loff,
+                                              typename C2<T>::t *__restrict__ W1) {
+  using V = typename C2<T>::t;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2];
+  V *cols = reinterpret_cast<V *>(smem);
+  V *tw = cols + (size_t)n2 * tx2;
+  const int x0 = blockIdx.x * tx2, k3 = blockIdx.y, cf = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int lo = lo2[k3], hi = hi2[k3];
+  for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
+  for (int i = tid; i < n2 * tx2; i += NT) cols[i] = V{T(0), T(0)};
+  __syncthreads();
+  const V *src = W2 + ((size_t)cf * g.nlines + (hi >= lo ? loff[k3] : 0)) * n1;
+  for (int idx = tid; idx < (hi - lo + 1) * tx2; idx += NT) {
+    const int xl = idx % tx2, r = idx / tx2;
+    const int x = x0 + xl;
+    if (x >= n1) continue;
+    const int h2 = lo + r;
+    const int yi = h2 < 0 ? h2 + n2 : h2;
+    cols[yi * tx2 + xl] = src[(size_t)r * n1 + x];
+  }
+  __syncthreads();
+  fft_lines<T>(cols, tx2, ay, 1, tx2, tw, true);
+  V *dst = W1 + ((size_t)cf * (M3 + 1) + k3) * n2 * n1;
+  for (int idx = tid; idx < n2 * tx2; idx += NT) {
+    const int xl = idx % tx2, y = idx / tx2;
+    const int x = x0 + xl;
+    if (x >= n1) continue;
+    V v = cols[idx];
+    // the h3 = 0 plane holds only the canonical half of (h1, h2): the full
+    // plane sum is that half plus its conjugate
+    if (k3 == 0) v = V{T(2) * v.x, T(0)};
+    dst[(size_t)y * n1 + x] = v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
+                                                     const typename C2<T>::t *__restrict__ W1,
+                                                     const typename C2<T>::t *__restrict__ twz,
+                                                     const double *__restrict__ X,
+                                                     const int32_t *__restrict__ sidx,
+                                                     const int32_t *__restrict__ start,
+                                                     double *__restrict__ slots) {
+  using V = typename C2<T>::t;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2];
+  const int ccx = g.txr / 2, cc = ccx * g.ty;
+  V *tile = reinterpret_cast<V *>(smem);
+  V *tw = tile + (size_t)n3 * cc;
+  const T *tflat = reinterpret_cast<const T *>(tile);
+  const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx, cf = blockIdx.y;
+  const int x0 = tx * g.txr, y0 = ty * g.ty;
+  const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n3 * cc; i += NT) tile[i] = V{T(0), T(0)};
+  for (int i = tid; i < n3; i += NT) tw[i] = twz[i];
+  __syncthreads();
+  const int pw = txw / 2;
+  const V *src = W1 + (size_t)cf * (M3 + 1) * n2 * n1;
+  for (int idx = tid; idx < (M3 + 1) * tyw * pw; idx += NT) {
+    const int p = idx % pw;
+    const int r = idx / pw;
+    const int yl = r % tyw;
+    const int k3 = r / tyw;
+    const V A = src[((size_t)k3 * n2 + y0 + yl) * n1 + x0 + 2 * p];
+    const V B = src[((size_t)k3 * n2 + y0 + yl) * n1 + x0 + 2 * p + 1];
+    const int col = yl * ccx + p;
+    tile[(size_t)k3 * cc + col] = V{A.x - B.y, A.y + B.x};
+    if (k3 > 0) tile[(size_t)(n3 - k3) * cc + col] = V{A.x + B.y, B.x - A.y};
+  }
+  __syncthreads();
+  fft_lines<T>(tile, cc, az, 1, cc, tw, true);
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = NT / 64;
+  const int ns = g.ns;
+  const int S2 = g.sx * g.sy;
+  int bxs, bxn, bys, byn;
+  bin_window(tx, g.hx, g.ntx, &bxs, &bxn);
+  bin_window(ty, g.hy, g.nty, &bys, &byn);
+  int seen = 0;
+  for (int iby = 0; iby < byn; ++iby) {
+    const int by = wrapi(bys + iby, g.nty);
+    for (int ibx = 0; ibx < bxn; ++ibx) {
+      const int bx = wrapi(bxs + ibx, g.ntx);
+      const int bin = (cf * g.nty + by) * g.ntx + bx;
+      const int k0 = start[bin], k1 = start[bin + 1];
+      for (int k = k0; k < k1; ++k, ++seen) {
+        if (seen % nw != wave) continue;
+        const int j = sidx[k];
+        const double Xx = X[3 * j], Xy = X[3 * j + 1], Xz = X[3 * j + 2];
+        const int i0x = (int)ceil(Xx - g.half_ns);
+        const int i0y = (int)ceil(Xy - g.half_ns);
+        const int i0z = (int)ceil(Xz - g.half_ns);
+        int kxl, xll, kyl, yll;
+        const int ox = overlap(i0x, ns, n1, x0, txw, &kxl, &xll);
+        if (ox == 0) continue;
+        const int oy = overlap(i0y, ns, n2, y0, tyw, &kyl, &yll);
+        if (oy == 0) continue;
+        T phi = T(0), dphi = T(0);
+        if (lane < 3 * ns) {
+          const int ax = lane / ns, kk = lane - ax * ns;
+          const double X3 = ax == 0 ? Xx : (ax == 1 ? Xy : Xz);
+          const int i03 = ax == 0 ? i0x : (ax == 1 ? i0y : i0z);
+          phi = es_phi<T>((double)(i03 + kk) - X3, (double)ns, g.beta, &dphi);
+        }
+        const int a = wrapi(i0z, n3);
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const int cells = ox * oy * ns;
+        const int iters = (cells + 63) / 64;
+        for (int it = 0; it < iters; ++it) {
+          const int c = lane + 64 * it;
+          const int cv = c < cells ? c : 0;
+          const int ix = cv % ox;
+          const int t2 = cv / ox;
+          const int iy = t2 % oy;
+          const int kz = t2 / oy;
+          const T fx = __shfl(phi, kxl + ix), dfx = __shfl(dphi, kxl + ix);
+          const T fy = __shfl(phi, ns + kyl + iy), dfy = __shfl(dphi, ns + kyl + iy);
+          const T fz = __shfl(phi, 2 * ns + kz), dfz = __shfl(dphi, 2 * ns + kz);
+          if (c < cells) {
+            const int xl = xll + ix, yl = yll + iy;
+            int z = a + kz;
+            if (z >= n3) z -= n3;
+            const int col = yl * ccx + (xl >> 1);
+            const double v = (double)tflat[2 * ((size_t)z * cc + col) + (xl & 1)];
+            gx += v * (double)(dfx * fy * fz);
+            gy += v * (double)(fx * dfy * fz);
+            gz += v * (double)(fx * fy * dfz);
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          gx += __shfl_xor(gx, off);
+          gy += __shfl_xor(gy, off);
+          gz += __shfl_xor(gz, off);
+        }
+        if (lane == 0) {
+          const int xs = wrapi(i0x, n1), ys = wrapi(i0y, n2);
+          const int dx = wrapi(tx - xs / g.txr, g.ntx);
+          const int dy = wrapi(ty - ys / g.ty, g.nty);
+          double *o = slots + ((size_t)k * S2 + dx * g.sy + dy) * 3;
+          o[0] = gx;
+          o[1] = gy;
+          o[2] = gz;
+        }
+      }
+    }
+  }
+}
+
+// Sum each atom's slots in a fixed order and map grid derivatives to dL/dr:
+// dL/dr = sum_i b_i (n_i / 2 pi) dV/dX_i, dV/dX_i = -sum_g V phi'(u_i) prod phi.
+__global__ void k_adj_sum(Geo g, const double *__restrict__ X, const int32_t *__restrict__ sidx,
+                          const int32_t *__restrict__ start, const int32_t *__restrict__ species,
+                          const double *__restrict__ slots, int Fa, double *__restrict__ grad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.natoms) return;
+  const int j = sidx[k];
+  const double Xx = X[3 * j], Xy = X[3 * j + 1];
+  const int i0x = (int)ceil(Xx - g.half_ns), i0y = (int)ceil(Xy - g.half_ns);
+  const int xs = wrapi(i0x, g.n[0]), ys = wrapi(i0y, g.n[1]);
+  const int tx0 = xs / g.txr, ty0 = ys / g.ty;
+  const int S2 = g.sx * g.sy;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int dx = 0; dx < g.sx; ++dx) {
+    const int tx = wrapi(tx0 + dx, g.ntx);
+    const int x0 = tx * g.txr, txw = min(g.txr, g.n[0] - x0);
+    int a, b;
+    if (overlap(i0x, g.ns, g.n[0], x0, txw, &a, &b) == 0) continue;
+    if (dx > 0 && tx == tx0) break;
+    for (int dy = 0; dy < g.sy; ++dy) {
+      const int ty = wrapi(ty0 + dy, g.nty);
+      if (dy > 0 && ty == ty0) break;
+      const int y0 = ty * g.ty, tyw = min(g.ty, g.n[1] - y0);
+      if (overlap(i0y, g.ns, g.n[1], y0, tyw, &a, &b) == 0) continue;
+      const double *o = slots + ((size_t)k * S2 + dx * g.sy + dy) * 3;
+      s0 += o[0];
+      s1 += o[1];
+      s2 += o[2];
+    }
+  }
+  const double w = (Fa == 1) ? g.amp[species[j]] : 1.0;
+  const double inv_two_pi = 1.0 / (2.0 * 3.14159265358979323846);
+  const double d0 = -s0 * g.n[0] * inv_two_pi * w;
+  const double d1 = -s1 * g.n[1] * inv_two_pi * w;
+  const double d2 = -s2 * g.n[2] * inv_two_pi * w;
+  // columns of b are b_1, b_2, b_3
+#pragma unroll
+  for (int c = 0; c < 3; ++c) grad[3 * j + c] = g.b[c] * d0 + g.b[c + 3] * d1 + g.b[c + 6] * d2;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+#define SQC_CHECK(x)                         \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) return (int)e_;    \
+  } while (0)
+
+template <typename T>
+static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
+  using V = typename C2<T>::t;
+  Geo &g = p.geo;
+  hipStream_t st = p.stream;
+  const int nbins_tot = g.F * g.nbins;
+  const int blk = 256;
+  const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
+  hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
+                     p.d_count);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
+                     nbins_tot);
+  hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
+                     p.d_cursor, p.d_sidx);
+  hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
+                     p.d_start, p.d_sidx);
+  hipLaunchKernelGGL(k_spread_z<T>, dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2],
+                     p.d_X, p.d_sidx, p.d_start, p.d_species, (const V *)p.d_tw[2], (V *)p.d_W1);
+  hipLaunchKernelGGL(k_fft_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, g.F), dim3(NT),
+                     p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W1, (const V *)p.d_tw[1],
+                     p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W2);
+  const int ngroups = (g.nlines + p.lb - 1) / p.lb;
+  hipLaunchKernelGGL(k_fft_x_bin<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb,
+                     ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
+                     (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0], p.d_ph[1],
+                     p.d_ph[2], p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
+                     mode == 0 ? p.d_gnum : nullptr);
+  SQC_CHECK(hipGetLastError());
+  return launch_finalize(p, frame_out, mode == 0);
+}
+
+int launch_finalize(Plan &p, double *frame_out, int accumulate) {
+  const int ntot = p.geo.ncomp * p.geo.nq;
+  hipLaunchKernelGGL(k_finalize, dim3((ntot + 255) / 256), dim3(256), 0, p.stream, p.d_wgpart,
+                     p.nblk3, ntot, p.d_acc, frame_out, accumulate);
+  SQC_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_forward(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
+  return p.prec == 1 ? forward_t<float>(p, d_pos, mode, frame_out, rho_out)
+                     : forward_t<double>(p, d_pos, mode, frame_out, rho_out);
+}
+
+template <typename T>
+static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double *gS,
+                     const double *gP, double *grad) {
+  using V = typename C2<T>::t;
+  Geo &g = p.geo;
+  hipStream_t st = p.stream;
+  const int nbins_tot = g.F * g.nbins;
+  const int blk = 256;
+  const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
+  const int Fa = g.F;
+  // the atoms of this frame, sorted exactly as in the forward pass
+  hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
+                     p.d_count);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
+                     nbins_tot);
+  hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
+                     p.d_cursor, p.d_sidx);
+  hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
+                     p.d_start, p.d_sidx);
+  const int ngroups = (g.nlines + p.lb - 1) / p.lb;
+  hipLaunchKernelGGL(k_adj_x<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
+                     Fa, (const V *)rho, (int64_t)p.ncand, (const V *)p.d_tw[0],
+                     (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0], p.d_ph[1],
+                     p.d_ph[2], p.d_den, p.d_cnt, gS, gP, (V *)p.d_W2);
+  hipLaunchKernelGGL(k_adj_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, Fa), dim3(NT),
+                     p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1],
+                     p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W1);
+  hipLaunchKernelGGL(k_adj_z_gather<T>, dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1, st, g,
+                     p.ax[2], (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_X, p.d_sidx,
+                     p.d_start, p.d_slots);
+  hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_X, p.d_sidx, p.d_start,
+                     p.d_species, p.d_slots, Fa, grad);
+  SQC_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_adjoint(Plan &p, const double *d_pos, const void *rho, const double *gS,
+                   const double *gP, double *grad) {
+  return p.prec == 1 ? adjoint_t<float>(p, d_pos, rho, gS, gP, grad)
+                     : adjoint_t<double>(p, d_pos, rho, gS, gP, grad);
+}
+
+// Raise the dynamic-LDS ceiling of the kernels that use more than 64 KiB.
+void kernel_lds_limits(Plan &p) {
+  const int mx = 160 * 1024;
+  if (p.prec == 1) {
+    (void)hipFuncSetAttribute((const void *)k_spread_z<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_fft_y<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_fft_x_bin<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_adj_x<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_adj_y<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_adj_z_gather<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+  } else {
+    (void)hipFuncSetAttribute((const void *)k_spread_z<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_fft_y<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_fft_x_bin<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_adj_x<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_adj_y<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute((const void *)k_adj_z_gather<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+  }
+}
+
+}  // namespace sqc

@@ -1,0 +1,100 @@
+"""GPU parity: libsqc_hip against the FP64 oracle and the golden tables.
+
+Tolerances: the FP64 path within 1e-9 and the FP32 path within 1e-6 of the
+oracle, in the reference suite's metric max|d| / max(|ref|, 1)
+(test/test_sqcalc.py:102-110), column by column.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import (candidate_layout, golden_cases, load_golden, oracle_run, random_case,
+                     rel_dev)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"double": 1e-9, "single": 1e-6}
+
+
+def _hip(cell, origin, types, mapping, weight, norm, qmin, qmax, nq, precision, **kw):
+    from sqcalc_amd import HipStructureFactor
+    return HipStructureFactor(cell, origin, types, mapping, weight=weight, norm=norm, qmin=qmin,
+                              qmax=qmax, nq=nq, precision=precision, **kw)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("name", golden_cases())
+def test_golden_tables(hip_device, name, precision):
+    d = load_golden(name)
+    print("case", name, precision, flush=True)
+    sf = _hip(d["cell"], d["origin"], d["types"], d["mapping_dict"], d["weight"], d["norm"],
+              d["qmin"], d["qmax"], d["nq"], precision, eps=1e-10 if precision == "double" else None)
+    sf.accumulate(d["frames"])
+    table = sf.table()
+    sf.close()
+    ref = oracle_run(d["cell"], d["origin"], d["types"], d["mapping_dict"], d["weight"],
+                     d["norm"], d["qmin"], d["qmax"], d["nq"], d["frames"]).table()
+    np.testing.assert_allclose(table[:, 0], ref[:, 0], rtol=0, atol=1e-12)
+    dev_golden = rel_dev(np.array(d["S"]), table[:, 1])
+    devs = [rel_dev(ref[:, c], table[:, c]) for c in range(1, ref.shape[1])]
+    print("  golden %.2e oracle columns %s" % (dev_golden, ["%.2e" % v for v in devs]))
+    assert dev_golden <= max(TOL[precision], 2e-12)
+    assert max(devs) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("tilt", [None, (1.5, 0.5, -2.0)])
+def test_mode_amplitudes(hip_device, precision, tilt):
+    """rho_a(q) at every canonical kept mode against the direct sum."""
+    import torch
+    cell, origin, types, frames = random_case(natoms=400, length=17.0, nframes=1, seed=3,
+                                              tilt=tilt)
+    qmax, nq = 5.0, 50
+    print("case amplitudes", precision, tilt, flush=True)
+    # per-mode error is set by the NUFFT tolerance (relative to max |rho|)
+    eps = 1e-10 if precision == "double" else 1e-6
+    sf = _hip(cell, origin, types, None, "unit", "mean", 0.0, qmax, nq, precision, eps=eps)
+    info = sf.info
+    pos = torch.tensor(frames, dtype=torch.float64, device=hip_device).contiguous()
+    ncomp = info.ncomp
+    out = torch.zeros((1, ncomp, nq), dtype=torch.float64, device=hip_device)
+    cdt = torch.complex64 if precision == "single" else torch.complex128
+    rho = torch.zeros((1, info.nfields, info.rho_len), dtype=cdt, device=hip_device)
+    rc = sf.lib.sqc_hip_forward_device(sf.plan, pos.data_ptr(), 1, out.data_ptr(),
+                                       rho.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    rho = rho.cpu().numpy()[0]
+    md = O.modes(cell, 0.0, qmax, nq)
+    slots = candidate_layout(md)
+    assert len(slots) == info.rho_len
+    idx = {tuple(h): i for i, h in enumerate(md["hkl"].tolist())}
+    sel = [i for i, h in enumerate(slots) if h is not None]
+    assert len(sel) == info.nhalf
+    ref = O.direct_rho(frames[0], cell, origin, np.searchsorted(np.unique(types), types), 2, md)
+    refsel = ref[:, [idx[slots[i]] for i in sel]]
+    got = rho[:, sel]
+    scale = np.abs(refsel).max()
+    err = np.abs(got - refsel).max() / scale
+    print("  max |d rho| / max |rho| = %.2e" % err)
+    assert err <= (5e-10 if precision == "double" else 2e-6)
+    sf.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_random_partials_xray(hip_device, precision):
+    cell, origin, types, frames = random_case(natoms=500, length=18.0, nframes=3, seed=9,
+                                              ntypes=3, tilt=(2.0, -1.0, 0.5))
+    mapping = {1: "Si", 2: "O", 3: "Na"}
+    print("case xray partials", precision, flush=True)
+    sf = _hip(cell, origin, types, mapping, "xray", "mean", 0.3, 5.5, 60, precision)
+    sf.accumulate(frames)
+    table = sf.table()
+    sf.close()
+    ref = oracle_run(cell, origin, types, mapping, "xray", "mean", 0.3, 5.5, 60, frames).table()
+    devs = [rel_dev(ref[:, c], table[:, c]) for c in range(1, ref.shape[1])]
+    print("  columns", ["%.2e" % v for v in devs])
+    assert max(devs) <= TOL[precision]
