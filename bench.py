@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the reciprocal-space S(q) path on one MI355X.
+
+Workload (the metric configuration): cubic box L = 113 A, N = 96,000 atoms
+(32,000 Si + 64,000 O) on a seeded subset of the 46^3 simple-cubic lattice
+with N(0, 0.3^2) jitter per frame, neutron weights, partials on, q_max 10,
+500 shells.  Positions are generated from a seed and resident in HBM before
+the timed region.
+
+A step is one pass of the path over a batch of --frames-per-step frames.
+The headline value is the FP32-spread forward (S(q) + partials, FP64 shell
+sums) in frames/s.  Next to it the line reports the plain FP64 (sigma = 2)
+forward, the forward + adjoint of the S(q)-matching loss on the same
+geometry, per-stage times from HIP events, the HBM roofline of the dominant
+kernel and the CPU restatement of the reference FINUFFT path timed on the
+host cores.
+
+  python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--frames-per-step 10]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames-per-step", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip forward + adjoint")
+    ap.add_argument("--fp64-frames", type=int, default=10)
+    ap.add_argument("--adj-frames", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(frames would shard across GPUs with one all-reduce of the shell sums)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from sqcalc_amd import HipStructureFactor
+    from sqcalc_amd.synthetic import config
+
+    dev = torch.device("cuda:0")
+    wl = config("c2")
+    fps = args.frames_per_step
+    nbatch = max(1, wl.nframes // fps)
+    print("# generating %d frames of %d atoms" % (nbatch * fps, wl.natoms), file=sys.stderr,
+          flush=True)
+    host = wl.frames(0, nbatch * fps)
+    pos = torch.tensor(host, dtype=torch.float64, device=dev).contiguous()
+
+    def make(precision, **kw):
+        return HipStructureFactor(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight,
+                                  norm=wl.norm, qmin=wl.qmin, qmax=wl.qmax, nq=wl.nq,
+                                  precision=precision, **kw)
+
+    # ---- FP32-spread forward (headline) ----
+    sf = make("single")
+    info = sf.info
+    print("# fp32 plan: m=%s fine=%s ns=%d sigma=%.2f K=%d lines=%d tiles=%s bytes=%.2f GB"
+          % (list(info.modes), list(info.fine), info.nspread, info.upsampfac, info.nmodes,
+             info.nlines, list(info.tile), info.device_bytes / 1e9), file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        sf.accumulate_device(pos[(w % nbatch) * fps:(w % nbatch + 1) * fps])
+    torch.cuda.synchronize()
+    sf.fetch()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        b = k % nbatch
+        sf.accumulate_device(pos[b * fps:(b + 1) * fps])
+    sf.fetch()  # synchronises the plan stream
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    frames_per_s = fps * args.steps / (t1 - t0)
+
+    # per-stage event timing on the plan stream, one extra step
+    sf.profile(True)
+    sf.accumulate_device(pos[0:fps])
+    stages = sf.stage_times()
+    sf.profile(False)
+    nbytes = sf.stage_bytes()
+    dom = max((k for k in stages if k in nbytes), key=lambda k: stages[k][0])
+    dom_ms = stages[dom][0] / stages[dom][1]
+    achieved = nbytes[dom] / (dom_ms * 1e-3) / 1e9
+    traffic = _committed_traffic(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes": int(nbytes[dom]),
+                "avg_ms": round(dom_ms, 4)}
+    stage_ms = {k: round(v[0] / v[1], 4) for k, v in stages.items()}
+    # the whole frame against the same byte model
+    frame_bytes = sum(nbytes[k] for k in ("spread_z", "dft_y", "dft_x_bin"))
+    sf.close()
+
+    # ---- plain FP64 forward (sigma 2) ----
+    fp64 = None
+    if not args.no_fp64:
+        sd = make("double")
+        nf = min(args.fp64_frames, pos.shape[0])
+        sd.accumulate_device(pos[0:1])
+        sd.fetch()
+        t0 = time.perf_counter()
+        sd.accumulate_device(pos[0:nf])
+        sd.fetch()
+        fp64 = nf / (time.perf_counter() - t0)
+        fine64 = list(sd.info.fine)
+        sd.close()
+
+    # ---- forward + adjoint (S(q)-matching loss) ----
+    adj = None
+    if not args.no_adjoint:
+        from sqcalc_amd.autograd import sq_matching_loss
+        sa = make("single", partials=False)
+        tgt_sf = make("single", partials=False)
+        tgt_sf.accumulate_device(pos[-1:])
+        num, _, _, _ = tgt_sf.fetch()
+        tgt_sf.close()
+        # target: S(q) of a second seed's frame
+        target = torch.tensor(np.where(sa.den > 0, num / np.where(sa.den > 0, sa.den, 1), 0.0),
+                              device=dev)
+        nf = min(args.adj_frames, pos.shape[0])
+        frames = [pos[k].clone().requires_grad_(True) for k in range(nf)]
+        loss = sq_matching_loss(frames[0], sa, target)
+        loss.backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nf):
+            loss = sq_matching_loss(frames[k], sa, target)
+            loss.backward()
+        torch.cuda.synchronize()
+        adj = nf / (time.perf_counter() - t0)
+        sa.close()
+
+    # ---- CPU restatement of the reference FINUFFT path ----
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(wl, host[0])
+
+    line = {
+        "metric": "S(q) frames/s (fwd, FP32 spread, FP64 shell sums) at 96k atoms",
+        "value": round(frames_per_s, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered SiO2 lattice, positions resident in HBM)",
+        "config": {"workload": "c2: cubic L=113 A, N=96000 (Si 32000, O 64000), lattice + "
+                               "N(0,0.3^2) jitter, neutron, partials, q_max 10, nq 500",
+                   "frames_per_step": fps, "natoms": wl.natoms, "modes": list(info.modes),
+                   "kept_modes": int(info.nmodes), "fine_grid": list(info.fine),
+                   "ns": int(info.nspread), "upsampfac": info.upsampfac},
+        "roofline": roofline,
+        "frame_hbm_model_gb": round(frame_bytes / 1e9, 3),
+        "frame_frac_of_hbm_peak": round(frame_bytes * frames_per_s / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": stage_ms,
+        "fp64_plain_frames_per_s": round(fp64, 2) if fp64 else None,
+        "fp32_over_fp64": round(frames_per_s / fp64, 2) if fp64 else None,
+        "fwd_adj_frames_per_s": round(adj, 2) if adj else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _committed_traffic(kernel):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary (profiles/), when one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def _cpu_baseline(wl, frame):
+    """One frame of the CPU restatement of the reference FINUFFT path (sigma
+    1.25, ES kernel, eps 1e-9, FP64, scipy pocketfft), all host cores."""
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    run = O.Run(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight, norm=wl.norm,
+                qmin=wl.qmin, qmax=wl.qmax, nq=wl.nq)
+    t0 = time.perf_counter()
+    O.nufft_frame(run, frame, eps=1e-9, upsampfac=1.25, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": "1 frame of the same workload (FP64, sigma 1.25, eps 1e-9, %.1f s)" % dt}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

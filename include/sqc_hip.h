@@ -140,7 +140,9 @@ int sqc_hip_reset(sqc_hip_plan *plan);
  *   out_device[f][1 + npairs][nq]  num, then the a<=b pair sums
  *   rho_device (optional) the species amplitudes at the canonical kept modes,
  *              [f][nfields][rho_len] complex of the plan precision.
- * stream: a hipStream_t to order against (NULL = plan stream only). */
+ * stream: the caller's hipStream_t.  The work runs on the plan's stream,
+ * ordered after everything already queued on `stream` and before anything
+ * queued on it later; NULL is the legacy default stream. */
 int sqc_hip_forward_device(sqc_hip_plan *plan, const double *pos_device, int64_t nframes,
                            double *out_device, void *rho_device, void *stream);
 
@@ -158,6 +160,16 @@ int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const v
  * frame, host memory in and out (target[nq], wq[nq], grad[3*natoms]). */
 int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target,
                      const double *wq, double *loss, double *grad);
+
+/* Per-stage timing with HIP events on the launch stream (no counterpart in
+ * the reference, whose only timing is the driver's wall clock,
+ * app/main.f90:218-220).  Stages: 0 atom wrap+sort, 1 spread+z-DFT, 2 y-DFT,
+ * 3 x-DFT+shell sums, 4 finalize, 5 adjoint x, 6 adjoint y, 7 adjoint
+ * z+gather, 8 adjoint slot sum.  sqc_hip_stage_times synchronises, returns
+ * the milliseconds and launch counts accumulated since the last call, and
+ * clears them. */
+int sqc_hip_profile(sqc_hip_plan *plan, int enable);
+int sqc_hip_stage_times(sqc_hip_plan *plan, double *ms, int64_t *launches, int nstage);
 
 int sqc_hip_destroy(sqc_hip_plan *plan);
 

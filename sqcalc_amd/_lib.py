@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "libsqc_hip.so")
 ABI_VERSION = 1
 F64, F32 = 0, 1
 WEIGHT = {"unit": 0, "neutron": 1, "xray": 2}
+STAGES = ["atom_sort", "spread_z", "dft_y", "dft_x_bin", "finalize", "adj_x", "adj_y",
+          "adj_z_gather", "adj_sum"]
 NORM = {"mean": 1, "self": 2, "n": 3}
 
 _c = ctypes
@@ -57,6 +59,8 @@ SIGNATURES = {
     "sqc_hip_forward_device": (_c.c_int, [_vp, _vp, _c.c_int64, _vp, _vp, _vp]),
     "sqc_hip_adjoint_device": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sqc_hip_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sqc_hip_profile": (_c.c_int, [_vp, _c.c_int]),
+    "sqc_hip_stage_times": (_c.c_int, [_vp, _vp, _vp, _c.c_int]),
     "sqc_hip_destroy": (_c.c_int, [_vp]),
     "sqc_hip_error_string": (_c.c_char_p, [_c.c_int]),
     "sqc_hip_last_error": (_c.c_char_p, []),

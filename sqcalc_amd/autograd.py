@@ -1,0 +1,83 @@
+"""torch.autograd entry point: S(q) of one frame and its adjoint on the GPU.
+
+``structure_factor(pos, sf)`` returns (S, P) for positions ``pos`` (N, 3)
+float64 on the device: S[nq] = num / den (one frame, the normalisation of
+src/sqc_structure_factor.f90:657-665) and P[npairs, nq] the pair columns
+pair / (N count) (:696-719, OVITO convention).  Backward runs the
+hand-written adjoint of libsqc_hip (type-2 transform of the shell weights
+with the kernel derivative); only the species amplitudes at the kept modes
+are saved between the passes, the fine grid is recomputed.  There is no
+counterpart in the reference, which has no gradient.
+"""
+
+import torch
+
+from . import _lib
+
+
+def _stream():
+    return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+
+def ctypes_void(v):
+    import ctypes
+    return ctypes.c_void_p(v)
+
+
+class _SqFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, sf):
+        if pos.dtype != torch.float64 or not pos.is_cuda:
+            raise TypeError("positions must be a float64 tensor on the HIP device")
+        pos = pos.contiguous()
+        info = sf.info
+        nq = sf.nq
+        out = torch.empty((info.ncomp, nq), dtype=torch.float64, device=pos.device)
+        cdt = torch.complex64 if sf.precision == "single" else torch.complex128
+        rho = torch.empty((info.nfields, info.rho_len), dtype=cdt, device=pos.device)
+        rc = sf.lib.sqc_hip_forward_device(sf.plan, pos.data_ptr(), 1, out.data_ptr(),
+                                           rho.data_ptr(), _stream())
+        _lib.check(rc, "sqc_hip_forward_device")
+        den = torch.as_tensor(sf.den, device=pos.device)
+        cnt = torch.as_tensor(sf.shell_count, device=pos.device, dtype=torch.float64)
+        S = torch.where(den > 0, out[0] / torch.where(den > 0, den, 1.0), 0.0)
+        P = out[1:] / (sf.natoms * torch.clamp(cnt, min=1.0))
+        ctx.sf = sf
+        ctx.save_for_backward(pos, rho)
+        return S, P
+
+    @staticmethod
+    def backward(ctx, gS, gP):
+        pos, rho = ctx.saved_tensors
+        sf = ctx.sf
+        nq = sf.nq
+        gS = (gS if gS is not None else torch.zeros(nq, dtype=torch.float64,
+                                                    device=pos.device)).contiguous()
+        gP_full = None
+        if gP is not None and sf.partials and bool(torch.any(gP != 0)):
+            S = len(sf.species_type)
+            gP_full = torch.zeros((S, S, nq), dtype=torch.float64, device=pos.device)
+            k = 0
+            for a in range(S):
+                for b in range(a, S):
+                    gP_full[a, b] = gP[k]
+                    k += 1
+        grad = torch.empty_like(pos)
+        rc = sf.lib.sqc_hip_adjoint_device(sf.plan, pos.data_ptr(), rho.data_ptr(),
+                                           gS.data_ptr(),
+                                           gP_full.data_ptr() if gP_full is not None else None,
+                                           grad.data_ptr(), _stream())
+        _lib.check(rc, "sqc_hip_adjoint_device")
+        return grad, None
+
+
+def structure_factor(pos, sf):
+    """(S, P) of one frame; differentiable with respect to ``pos``."""
+    return _SqFunction.apply(pos, sf)
+
+
+def sq_matching_loss(pos, sf, target, wq=None):
+    """L = sum_s w_s (S_s - T_s)^2 for one frame (the loss of the gradient workload)."""
+    S, _ = structure_factor(pos, sf)
+    w = torch.ones_like(S) if wq is None else wq
+    return torch.sum(w * (S - target) ** 2)

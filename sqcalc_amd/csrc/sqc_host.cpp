@@ -260,6 +260,8 @@ void free_plan(sqc_hip_plan *h) {
                   h->g_grad, h->g_rho};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  if (p.ev_in) (void)hipEventDestroy(p.ev_in);
+  if (p.ev_out) (void)hipEventDestroy(p.ev_out);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
 }
@@ -512,6 +514,8 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   p.nblk3 = std::max(1, std::min(ngroups, 2 * prop.multiProcessorCount));
   HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking), "hipStreamCreate");
   p.stream = h->own_stream;
+  HIP_TRY(hipEventCreateWithFlags(&p.ev_in, hipEventDisableTiming), "hipEventCreate");
+  HIP_TRY(hipEventCreateWithFlags(&p.ev_out, hipEventDisableTiming), "hipEventCreate");
 
   // --- device buffers ---
   const int64_t N = cfg->natoms;
@@ -566,13 +570,20 @@ int check_plan(const sqc_hip_plan *plan) {
   return 0;
 }
 
-struct StreamScope {
+// Orders the plan stream after the caller's stream on entry and the caller's
+// stream after the plan stream on exit.  The caller's stream may be the
+// legacy default stream (handle 0), which is what torch uses by default.
+struct OrderScope {
   sqc::Plan &p;
-  hipStream_t saved;
-  StreamScope(sqc::Plan &pl, void *s) : p(pl), saved(pl.stream) {
-    if (s) p.stream = (hipStream_t)s;
+  hipStream_t user;
+  OrderScope(sqc::Plan &pl, void *s) : p(pl), user((hipStream_t)s) {
+    (void)hipEventRecord(p.ev_in, user);
+    (void)hipStreamWaitEvent(p.stream, p.ev_in, 0);
   }
-  ~StreamScope() { p.stream = saved; }
+  ~OrderScope() {
+    (void)hipEventRecord(p.ev_out, p.stream);
+    (void)hipStreamWaitEvent(user, p.ev_out, 0);
+  }
 };
 
 }  // namespace
@@ -812,7 +823,7 @@ int sqc_hip_forward_device(sqc_hip_plan *plan, const double *pos_device, int64_t
     return fail(SQC_HIP_ERR_BAD_ARG, "bad forward arguments");
   sqc::Plan &p = plan->p;
   HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
-  StreamScope scope(p, stream);
+  OrderScope scope(p, stream);
   const int64_t stride = 3 * p.geo.natoms;
   const size_t vs = p.prec == 1 ? 8 : 16;
   const size_t rstride = (size_t)p.geo.F * p.ncand * vs;
@@ -839,7 +850,7 @@ int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const v
     int rc = alloc(&p.d_slots, sizeof(double) * 3 * p.geo.natoms * p.geo.sx * p.geo.sy, p.bytes);
     if (rc) return rc;
   }
-  StreamScope scope(p, stream);
+  OrderScope scope(p, stream);
   const int rc = sqc::launch_adjoint(p, pos_device, rho_device, gS_device, gP_device, grad_device);
   if (rc) return hip_fail((hipError_t)rc, "adjoint launch");
   return SQC_HIP_OK;
@@ -864,7 +875,7 @@ int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target
   HIP_TRY(hipMemcpyAsync(plan->g_pos, pos, sizeof(double) * 3 * g.natoms, hipMemcpyHostToDevice,
                          p.stream),
           "upload positions");
-  if ((rc = sqc_hip_forward_device(plan, plan->g_pos, 1, plan->g_out, plan->g_rho, nullptr)))
+  if ((rc = sqc_hip_forward_device(plan, plan->g_pos, 1, plan->g_out, plan->g_rho, p.stream)))
     return rc;
   std::vector<double> num(g.nq), gs(g.nq);
   HIP_TRY(hipMemcpyAsync(num.data(), plan->g_out, sizeof(double) * g.nq, hipMemcpyDeviceToHost,
@@ -884,7 +895,7 @@ int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target
                          p.stream),
           "upload gS");
   if ((rc = sqc_hip_adjoint_device(plan, plan->g_pos, plan->g_rho, plan->g_gs, nullptr,
-                                   plan->g_grad, nullptr)))
+                                   plan->g_grad, p.stream)))
     return rc;
   HIP_TRY(hipMemcpyAsync(grad, plan->g_grad, sizeof(double) * 3 * g.natoms,
                          hipMemcpyDeviceToHost, p.stream),
@@ -893,10 +904,36 @@ int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target
   return SQC_HIP_OK;
 }
 
+int sqc_hip_profile(sqc_hip_plan *plan, int enable) {
+  if (int rc = check_plan(plan)) return rc;
+  sqc::Plan &p = plan->p;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  const int e = sqc::collect_stage_times(p);
+  if (e) return hip_fail((hipError_t)e, "stage timing");
+  p.profiling = enable ? 1 : 0;
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_stage_times(sqc_hip_plan *plan, double *ms, int64_t *launches, int nstage) {
+  if (int rc = check_plan(plan)) return rc;
+  sqc::Plan &p = plan->p;
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  const int e = sqc::collect_stage_times(p);
+  if (e) return hip_fail((hipError_t)e, "stage timing");
+  for (int i = 0; i < nstage && i < sqc::kNumStages; ++i) {
+    if (ms) ms[i] = p.stage_ms[i];
+    if (launches) launches[i] = p.stage_n[i];
+    p.stage_ms[i] = 0.0;
+    p.stage_n[i] = 0;
+  }
+  return SQC_HIP_OK;
+}
+
 int sqc_hip_destroy(sqc_hip_plan *plan) {
   if (!plan) return SQC_HIP_OK;
   (void)hipSetDevice(plan->p.device);
   (void)hipStreamSynchronize(plan->p.stream);
+  (void)sqc::collect_stage_times(plan->p);
   free_plan(plan);
   return SQC_HIP_OK;
 }

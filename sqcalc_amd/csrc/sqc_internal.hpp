@@ -25,6 +25,9 @@ namespace sqc {
 constexpr int kMaxStages = 12;
 constexpr int kMaxSpecies = 8;
 constexpr uint16_t kNoShell = 0xFFFF;
+// timed stages: 0 atom sort, 1 spread+z, 2 y, 3 x+bin, 4 finalize,
+// 5 adjoint x, 6 adjoint y, 7 adjoint z+gather, 8 adjoint slot sum
+constexpr int kNumStages = 9;
 
 // Mixed-radix factorisation of one axis.
 struct AxisPlan {
@@ -63,6 +66,7 @@ struct Plan {
   sqc_hip_config cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   int prec = 0;               // 0 f64, 1 f32
   Geo geo{};
   int64_t m[3]{};
@@ -105,6 +109,15 @@ struct Plan {
   double *d_gs = nullptr;
   void *d_rho = nullptr;
   int64_t bytes = 0;
+  // per-stage timing (events recorded on the launch stream when enabled)
+  int profiling = 0;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  double stage_ms[kNumStages] = {};
+  int64_t stage_n[kNumStages] = {};
 };
 
 // Launchers (sqc_kernels.hip).  All run on p.stream.
@@ -114,5 +127,6 @@ int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const do
                    const double *gP, double *grad);
 int launch_finalize(Plan &p, double *frame_out, int accumulate);
 void kernel_lds_limits(Plan &p);
+int collect_stage_times(Plan &p);
 
 }  // namespace sqc

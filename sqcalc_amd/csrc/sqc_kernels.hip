@@ -922,6 +922,42 @@ __global__ void k_adj_sum(Geo g, const double *__restrict__ X, const int32_t *__
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// Records a pair of events around one stage when the plan is profiling.
+struct StageTimer {
+  Plan &p;
+  int stage;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(Plan &pl, int st) : p(pl), stage(st) {
+    if (!p.profiling) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      a = b = nullptr;
+      return;
+    }
+    (void)hipEventRecord(a, p.stream);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    (void)hipEventRecord(b, p.stream);
+    p.pending.push_back({stage, a, b});
+  }
+};
+
+int collect_stage_times(Plan &p) {
+  if (p.pending.empty()) return 0;
+  hipError_t e = hipStreamSynchronize(p.stream);
+  for (auto &q : p.pending) {
+    float ms = 0.f;
+    if (e == hipSuccess && hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) {
+      p.stage_ms[q.stage] += ms;
+      p.stage_n[q.stage] += 1;
+    }
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  p.pending.clear();
+  return (int)e;
+}
+
 #define SQC_CHECK(x)                         \
   do {                                       \
     hipError_t e_ = (x);                     \
@@ -936,31 +972,45 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
   const int nbins_tot = g.F * g.nbins;
   const int blk = 256;
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
-  hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
-                     p.d_count);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
-                     nbins_tot);
-  hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
-                     p.d_cursor, p.d_sidx);
-  hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
-                     p.d_start, p.d_sidx);
-  hipLaunchKernelGGL(k_spread_z<T>, dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2],
-                     p.d_X, p.d_sidx, p.d_start, p.d_species, (const V *)p.d_tw[2], (V *)p.d_W1);
-  hipLaunchKernelGGL(k_fft_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, g.F), dim3(NT),
-                     p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W1, (const V *)p.d_tw[1],
-                     p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W2);
+  {
+    StageTimer t(p, 0);
+    hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
+                       p.d_count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
+                       nbins_tot);
+    hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
+                       p.d_cursor, p.d_sidx);
+    hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
+                       p.d_start, p.d_sidx);
+  }
+  {
+    StageTimer t(p, 1);
+    hipLaunchKernelGGL(k_spread_z<T>, dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2],
+                       p.d_X, p.d_sidx, p.d_start, p.d_species, (const V *)p.d_tw[2],
+                       (V *)p.d_W1);
+  }
+  {
+    StageTimer t(p, 2);
+    hipLaunchKernelGGL(k_fft_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, g.F),
+                       dim3(NT), p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W1,
+                       (const V *)p.d_tw[1], p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W2);
+  }
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
-  hipLaunchKernelGGL(k_fft_x_bin<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb,
-                     ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
-                     (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0], p.d_ph[1],
-                     p.d_ph[2], p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
-                     mode == 0 ? p.d_gnum : nullptr);
+  {
+    StageTimer t(p, 3);
+    hipLaunchKernelGGL(k_fft_x_bin<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb,
+                       ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
+                       (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0],
+                       p.d_ph[1], p.d_ph[2], p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
+                       mode == 0 ? p.d_gnum : nullptr);
+  }
   SQC_CHECK(hipGetLastError());
   return launch_finalize(p, frame_out, mode == 0);
 }
 
 int launch_finalize(Plan &p, double *frame_out, int accumulate) {
   const int ntot = p.geo.ncomp * p.geo.nq;
+  StageTimer t(p, 4);
   hipLaunchKernelGGL(k_finalize, dim3((ntot + 255) / 256), dim3(256), 0, p.stream, p.d_wgpart,
                      p.nblk3, ntot, p.d_acc, frame_out, accumulate);
   SQC_CHECK(hipGetLastError());
@@ -983,27 +1033,42 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
   const int Fa = g.F;
   // the atoms of this frame, sorted exactly as in the forward pass
-  hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
-                     p.d_count);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
-                     nbins_tot);
-  hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
-                     p.d_cursor, p.d_sidx);
-  hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
-                     p.d_start, p.d_sidx);
+  {
+    StageTimer t(p, 0);
+    hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
+                       p.d_count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
+                       nbins_tot);
+    hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
+                       p.d_cursor, p.d_sidx);
+    hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
+                       p.d_start, p.d_sidx);
+  }
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
-  hipLaunchKernelGGL(k_adj_x<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
-                     Fa, (const V *)rho, (int64_t)p.ncand, (const V *)p.d_tw[0],
-                     (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0], p.d_ph[1],
-                     p.d_ph[2], p.d_den, p.d_cnt, gS, gP, (V *)p.d_W2);
-  hipLaunchKernelGGL(k_adj_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, Fa), dim3(NT),
-                     p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1],
-                     p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W1);
-  hipLaunchKernelGGL(k_adj_z_gather<T>, dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1, st, g,
-                     p.ax[2], (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_X, p.d_sidx,
-                     p.d_start, p.d_slots);
-  hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_X, p.d_sidx, p.d_start,
-                     p.d_species, p.d_slots, Fa, grad);
+  {
+    StageTimer t(p, 5);
+    hipLaunchKernelGGL(k_adj_x<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
+                       Fa, (const V *)rho, (int64_t)p.ncand, (const V *)p.d_tw[0],
+                       (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0], p.d_ph[1],
+                       p.d_ph[2], p.d_den, p.d_cnt, gS, gP, (V *)p.d_W2);
+  }
+  {
+    StageTimer t(p, 6);
+    hipLaunchKernelGGL(k_adj_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, Fa), dim3(NT),
+                       p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1],
+                       p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W1);
+  }
+  {
+    StageTimer t(p, 7);
+    hipLaunchKernelGGL(k_adj_z_gather<T>, dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1, st, g,
+                       p.ax[2], (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_X, p.d_sidx,
+                       p.d_start, p.d_slots);
+  }
+  {
+    StageTimer t(p, 8);
+    hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_X, p.d_sidx, p.d_start,
+                       p.d_species, p.d_slots, Fa, grad);
+  }
   SQC_CHECK(hipGetLastError());
   return 0;
 }

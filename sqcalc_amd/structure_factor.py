@@ -163,6 +163,33 @@ class HipStructureFactor:
     def reset(self):
         _lib.check(self.lib.sqc_hip_reset(self.plan), "reset")
 
+    def profile(self, enable=True):
+        """Record HIP events around every stage launch (see stage_times)."""
+        _lib.check(self.lib.sqc_hip_profile(self.plan, 1 if enable else 0), "profile")
+
+    def stage_times(self):
+        """{stage: (total ms, launches)} since the last call (synchronises)."""
+        n = len(_lib.STAGES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        _lib.check(self.lib.sqc_hip_stage_times(self.plan, ms, cnt, n), "stage_times")
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.STAGES) if cnt[i]}
+
+    def stage_bytes(self):
+        """Algorithmic HBM bytes per launch of the three DFT stages: each reads
+        its input once and writes its output once."""
+        i = self.info
+        vs = 8 if self.precision == "single" else 16
+        n1, n2 = i.fine[0], i.fine[1]
+        M3 = (i.modes[2] - 1) // 2
+        F = i.nfields
+        w1 = F * (M3 + 1) * n2 * n1 * vs
+        w2 = F * i.nlines * n1 * vs
+        atoms = self.natoms * (3 * 8 + 4 + 4)
+        return {"spread_z": w1 + atoms, "dft_y": w1 + w2,
+                "dft_x_bin": w2 + 2 * i.rho_len, "adj_x": w2 + F * vs * i.rho_len,
+                "adj_y": w1 + w2, "adj_z_gather": w1 + atoms}
+
     # -- output (sf_shell_value, sf_partial_value, sf_write_results) ---------
     def table(self):
         num, part, _, nframes = self.fetch()

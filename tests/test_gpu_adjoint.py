@@ -1,0 +1,67 @@
+"""GPU adjoint: dL/dr from the autograd entry point against the FP64 oracle.
+
+The oracle's analytic gradient is itself pinned by central finite
+differences (tests/test_oracle.py).  Tolerance: max |g - g_ref| / max |g_ref|
+below 1e-8 (FP64 path) and 1e-5 (FP32 path); the FP32 bound is the
+gradient's relative accuracy target.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import random_case
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"double": 1e-8, "single": 1e-5}
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("weight,partials", [("neutron", False), ("unit", True),
+                                             ("xray", True), ("xray", False)])
+def test_adjoint_matches_oracle(hip_device, precision, weight, partials):
+    import torch
+
+    from sqcalc_amd import HipStructureFactor
+    from sqcalc_amd.autograd import structure_factor
+
+    cell, origin, types, frames = random_case(natoms=300, length=15.0, nframes=1, seed=21,
+                                              tilt=(1.0, -0.5, 0.3))
+    mapping = {1: "Si", 2: "O"}
+    qmax, nq = 4.5, 30
+    print("case adjoint", precision, weight, partials, flush=True)
+    eps = 1e-11 if precision == "double" else 1e-6
+    sf = HipStructureFactor(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax,
+                            nq=nq, precision=precision, partials=partials, eps=eps)
+    rng = np.random.default_rng(4)
+    target = torch.tensor(rng.random(nq), device=hip_device)
+    pos = torch.tensor(frames[0], device=hip_device, requires_grad=True)
+    S, P = structure_factor(pos, sf)
+    loss = torch.sum((S - target) ** 2)
+    wP = None
+    if partials:
+        wP = torch.tensor(rng.standard_normal(tuple(P.shape)), device=hip_device)
+        loss = loss + torch.sum(wP * P)
+    loss.backward()
+    grad = pos.grad.cpu().numpy()
+
+    run = O.Run(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax, nq=nq,
+                partials=partials)
+    rho = O.direct_rho(frames[0], cell, origin, run.species_of, 2, run.md, method=0)
+    num, _ = O.shell_sums(rho, run.md, weight, run.amp, run.xcoef)
+    Sref = np.where(run.den > 0, num / np.where(run.den > 0, run.den, 1), 0)
+    np.testing.assert_allclose(S.detach().cpu().numpy(), Sref, rtol=0,
+                               atol=1e-9 if precision == "double" else 1e-5)
+    gS = 2 * (Sref - target.cpu().numpy())
+    gP = None
+    if partials:
+        gP = np.zeros((2, 2, nq))
+        w = wP.cpu().numpy()
+        gP[0, 0], gP[0, 1], gP[1, 1] = w[0], w[1], w[2]
+    gref = O.gradient(frames[0], cell, origin, run.species_of, run.md, rho, weight, run.amp,
+                      run.xcoef, gS, run.den, run.shell_count, gP=gP)
+    err = np.abs(grad - gref).max() / np.abs(gref).max()
+    print("  max |dg| / max |g| = %.2e" % err)
+    assert err <= TOL[precision]
+    sf.close()
