@@ -8,7 +8,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsqc_hip.so")
+# SQC_HIP_LIBRARY selects another build of the library (A/B measurements)
+LIB_PATH = os.environ.get("SQC_HIP_LIBRARY", os.path.join(HERE, "libsqc_hip.so"))
 
 ABI_VERSION = 1
 F64, F32 = 0, 1

@@ -124,6 +124,12 @@ bool factor(int n, sqc::AxisPlan &ax) {
       n /= r;
     }
   }
+  // magic multipliers for j / Ns at every stage
+  uint64_t Ns = 1;
+  for (int s = 0; s < ax.nst; ++s) {
+    ax.magic[s] = Ns == 1 ? 0u : (uint32_t)(((1ULL << 32) + Ns - 1) / Ns);
+    Ns *= (uint64_t)ax.rad[s];
+  }
   return n == 1;
 }
 
@@ -252,7 +258,7 @@ double amplitude(const sqc::Geo &g, int sp, double q) {
 
 void free_plan(sqc_hip_plan *h) {
   sqc::Plan &p = h->p;
-  void *bufs[] = {p.d_pos, p.d_X, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
+  void *bufs[] = {p.d_pos, p.d_X, p.d_i0, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
                   p.d_species, p.d_W1, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
                   p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
                   p.d_line_off, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
@@ -452,10 +458,20 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
 
   // --- launch shapes ---
   const size_t vs = p.prec == 1 ? 8 : 16;
-  const int maxe = p.prec == 1 ? 8192 : 4096;
   const size_t lds_cap = 160 * 1024;
   const size_t lds_pref = 80 * 1024;  // two workgroups per CU
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+  for (int i = 0; i < 3; ++i)
+    if (g.n[i] > sqc::kMaxLine)
+      return fail(SQC_HIP_ERR_MAXNALLOC, "fine grid of " + std::to_string(g.n[i]) +
+                                             " points per axis exceeds the LDS DFT limit of " +
+                                             std::to_string(sqc::kMaxLine));
+  // LDS line strides: a z column stride of 24 words mod 64 spreads the
+  // neighbouring columns of a spread update over the banks; a y line stride
+  // of 2 words mod 64 does the same for the transposed loads
+  g.cs = n3 + ((12 - n3) % 32 + 32) % 32;
+  g.ls2 = n2 + ((1 - n2) % 32 + 32) % 32;
+  g.rs1 = n1;
   {
     static const int cand[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {4, 2}, {4, 1}, {2, 1}};
     int pick = -1;
@@ -463,17 +479,20 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
       for (int c = 0; c < 7; ++c) {
         const int txr = cand[c][0], ty = cand[c][1];
         const int cc = (txr / 2) * ty;
-        const size_t lds = ((size_t)cc * n3 + n3) * vs;
-        if (cc * n3 > maxe) continue;
+        const size_t lds = ((size_t)cc * g.cs + n3) * vs + sqc::kStageCap * (16 + 2 * 4);
         if (lds > (pass == 0 ? lds_pref : lds_cap)) continue;
         if (2 * txr > n1 || 2 * ty > n2) continue;
+        const int ntx = (n1 + txr - 1) / txr, nty = (n2 + ty - 1) / ty;
+        const int hx = (int)std::ceil((0.5 * ns + 1.0) / txr) + (n1 % txr ? 1 : 0);
+        const int hy = (int)std::ceil((0.5 * ns + 1.0) / ty) + (n2 % ty ? 1 : 0);
+        if (std::min(2 * hx + 1, ntx) * std::min(2 * hy + 1, nty) > 128) continue;
         pick = c;
         break;
       }
-    if (pick < 0) return fail(SQC_HIP_ERR_MAXNALLOC, "z line too long for the LDS spreader");
+    if (pick < 0) return fail(SQC_HIP_ERR_MAXNALLOC, "no spread tile fits the LDS");
     g.txr = cand[pick][0];
     g.ty = cand[pick][1];
-    p.lds1 = ((size_t)(g.txr / 2) * g.ty * n3 + n3) * vs;
+    p.lds1 = ((size_t)(g.txr / 2) * g.ty * g.cs + n3) * vs + sqc::kStageCap * (16 + 2 * 4);
   }
   g.ntx = (n1 + g.txr - 1) / g.txr;
   g.nty = (n2 + g.ty - 1) / g.ty;
@@ -485,17 +504,17 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   g.sy = std::min(g.nty, (ns - 1 + g.ty - 1) / g.ty + 2);
   {
     int tx2 = 16;
-    while (tx2 > 1 && (tx2 * n2 > maxe || ((size_t)tx2 * n2 + n2) * vs > lds_pref)) tx2 /= 2;
-    if (tx2 * n2 > maxe) return fail(SQC_HIP_ERR_MAXNALLOC, "y line too long for the LDS DFT");
+    while (tx2 > 1 && ((size_t)tx2 * g.ls2 + n2) * vs > lds_pref) tx2 /= 2;
     p.tx2 = tx2;
-    p.lds2 = ((size_t)tx2 * n2 + n2) * vs;
+    p.lds2 = ((size_t)tx2 * g.ls2 + n2) * vs;
   }
   {
-    const size_t accb = (size_t)g.ncomp * g.nq * sizeof(double);
-    int lb = 8;
-    auto lds3 = [&](int l) { return ((size_t)l * g.F * n1 + n1) * vs + accb; };
-    while (lb > 1 && (lb * g.F * n1 > maxe || lds3(lb) > lds_pref)) --lb;
-    if (lb * g.F * n1 > maxe || lds3(lb) > lds_cap)
+    // shell accumulators and the 1 / phihat_1 table follow the rows
+    const size_t accb = ((size_t)g.ncomp * g.nq + g.M[0] + 1) * sizeof(double);
+    int lb = sqc::kMaxLb;
+    auto lds3 = [&](int l) { return ((size_t)l * g.F * g.rs1 + n1) * vs + accb; };
+    while (lb > 1 && lds3(lb) > lds_pref) --lb;
+    if (lds3(lb) > lds_cap)
       return fail(SQC_HIP_ERR_MAXNALLOC, "x line x species too large for the LDS DFT");
     p.lb = lb;
     p.lds3 = lds3(lb);
@@ -524,6 +543,9 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   p.batch_cap = cfg->max_batch > 0 ? cfg->max_batch : 8;
   if ((rc = alloc(&p.d_pos, sizeof(double) * 3 * N * p.batch_cap, p.bytes))) return rc;
   if ((rc = alloc(&p.d_X, sizeof(double) * 3 * N, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_i0, sizeof(int4) * N, p.bytes))) return rc;
+  if ((rc = alloc((char **)&p.d_phi, (vs / 2) * 3 * ns * N, p.bytes))) return rc;
+  if ((rc = alloc((char **)&p.d_dphi, (vs / 2) * 3 * ns * N, p.bytes))) return rc;
   if ((rc = alloc(&p.d_key, sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = alloc(&p.d_sidx, sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = upload(&p.d_species, p.species_of.data(), sizeof(int32_t) * N, p.bytes))) return rc;
@@ -560,7 +582,8 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
     if ((rc = alloc(&p.d_gnum, sizeof(double) * p.ncand, p.bytes))) return rc;
     HIP_TRY(hipMemset(p.d_gnum, 0, sizeof(double) * p.ncand), "hipMemset");
   }
-  sqc::kernel_lds_limits(p);
+  if (int e = sqc::kernel_lds_limits(p))
+    return hip_fail((hipError_t)e, "setting the LDS size of the kernels");
   HIP_TRY(hipDeviceSynchronize(), "plan set-up");
   return ret;
 }

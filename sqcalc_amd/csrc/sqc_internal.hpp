@@ -18,7 +18,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sqc_hip.h"
+#include "sqc_hip.h"
 
 namespace sqc {
 
@@ -34,7 +34,16 @@ struct AxisPlan {
   int n;
   int nst;
   int rad[kMaxStages];
+  // ceil(2^32 / Ns) of each stage (0 when Ns = 1): j / Ns = umulhi(j, magic)
+  uint32_t magic[kMaxStages];
 };
+
+// Longest line the wave-per-line DFT handles (fine-grid points per axis).
+constexpr int kMaxLine = 1024;
+// Most lines an x-pass workgroup transforms per step.
+constexpr int kMaxLb = 8;
+// Atoms staged into LDS per round of the spreader / gather.
+constexpr int kStageCap = 512;
 
 // Everything a kernel needs to know about the geometry, passed by value.
 struct Geo {
@@ -48,6 +57,9 @@ struct Geo {
   double half_ns;
   // spread tiles (real x-columns x rows) and the bins that sort atoms by tile
   int txr, ty, ntx, nty, nbins;
+  int cs;           // LDS stride of a z column in the spread tile (padded)
+  int ls2;          // LDS stride of a y line (padded)
+  int rs1;          // LDS stride of an x row (padded)
   int hx, hy;       // bin halo each side
   int sx, sy;       // adjoint gradient slots per atom per axis
   int F;            // fields
@@ -95,6 +107,9 @@ struct Plan {
   // device buffers
   double *d_pos = nullptr;
   double *d_X = nullptr;
+  int4 *d_i0 = nullptr;        // stencil start per atom (i0x, i0y, i0z, -)
+  void *d_phi = nullptr;       // kernel values per atom [3][ns], path precision
+  void *d_dphi = nullptr;      // kernel derivatives (adjoint)
   int32_t *d_key = nullptr, *d_count = nullptr, *d_start = nullptr, *d_cursor = nullptr,
           *d_sidx = nullptr, *d_species = nullptr;
   void *d_W1 = nullptr, *d_W2 = nullptr, *d_tw[3] = {nullptr, nullptr, nullptr};
@@ -126,7 +141,7 @@ int launch_forward(Plan &p, const double *d_pos_frame, int mode, double *frame_o
 int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const double *gS,
                    const double *gP, double *grad);
 int launch_finalize(Plan &p, double *frame_out, int accumulate);
-void kernel_lds_limits(Plan &p);
+int kernel_lds_limits(Plan &p);
 int collect_stage_times(Plan &p);
 
 }  // namespace sqc

@@ -1,11 +1,13 @@
 // sqc_kernels.hip -- gfx950 kernels of the reciprocal-space S(q) path.
 //
 // Forward, per frame (one launch each):
-//   k_prep        wrap the atoms into the cell (FP64), grid coordinates, tile bin
+//   k_prep        wrap the atoms into the cell (FP64), grid coordinates, stencil
+//                 start and the 3 x ns kernel values of every atom, tile bin
 //   k_scan        exclusive scan of the bin counts (one workgroup)
 //   k_scatter     counting-sort scatter of atom indices by bin
 //   k_binsort     order each bin by atom index (makes the spread deterministic)
-//   k_spread_z    per (x,y) tile: spread into LDS, z-DFT in LDS, write h3 >= 0
+//   k_spread_z    per (x,y) tile: stage the atoms that reach the tile in LDS,
+//                 spread them into LDS columns, z-DFT, write h3 >= 0
 //   k_fft_y       per (h3, x-chunk): y-DFT in LDS, write the canonical lines
 //   k_fft_x_bin   per group of lines: x-DFT in LDS, deconvolve, combine species,
 //                 shell sums in LDS, per-workgroup partials
@@ -14,9 +16,11 @@
 // k_adj_z_gather, k_adj_sum.
 //
 // All DFTs carry the +i sign of the reference transform
-// (src/sqc_structure_factor.f90:1069-1071, iflag = +1) and are mixed-radix
-// Stockham passes over lines held in LDS, with the radix-R butterflies of a
-// stage staged through registers so that the pass runs in place.
+// (src/sqc_structure_factor.f90:1069-1071, iflag = +1).  A line of n points
+// is transformed by one wavefront, in place in LDS: mixed-radix Stockham
+// stages whose butterflies are staged through registers; the lanes of a wave
+// execute in lockstep and LDS serves one wave in order, so a stage needs no
+// workgroup barrier.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -27,6 +31,7 @@
 namespace sqc {
 
 constexpr int NT = 256;  // threads per workgroup for every LDS kernel
+constexpr int NW = NT / 64;
 
 template <typename T> struct C2;
 template <> struct C2<float> { using t = float2; };
@@ -126,77 +131,68 @@ template <typename T> struct Dft<T, 8> {
   }
 };
 
-// Max LDS elements a workgroup transforms at once: each thread stages at
-// most ceil(MAXE / (R NT)) radix-R butterflies in registers.
-template <typename T> struct FftCap;
-template <> struct FftCap<float> { static constexpr int maxe = 8192; };
-template <> struct FftCap<double> { static constexpr int maxe = 4096; };
+// Keeps the compiler from moving LDS accesses across this point; the wave
+// itself already executes its LDS instructions in order.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-// One Stockham stage over nlines lines of length n.  Element (line, pos) is
-// lds[line * ls + pos * ps].  line_fast selects the thread order: lines
-// fastest (interleaved layout, ls = 1) or butterflies fastest (ps = 1).
+// One Stockham stage of a line of n points (contiguous in LDS) by one wave.
 template <typename T, int R>
-__device__ __forceinline__ void fft_stage(typename C2<T>::t *lds, int nlines, int n, int Ns, int ls,
-                                          int ps, const typename C2<T>::t *tw, bool line_fast) {
+__device__ __forceinline__ void wave_stage(typename C2<T>::t *line, int n, int Ns, uint32_t magic,
+                                           const typename C2<T>::t *tw, int lane) {
   using V = typename C2<T>::t;
-  constexpr int MAXB = (FftCap<T>::maxe + R * NT - 1) / (R * NT);
+  constexpr int MAXB = (kMaxLine + R * 64 - 1) / (R * 64);
   const int nb = n / R;
-  const int total = nlines * nb;
   const int tstep = n / (Ns * R);
-  const int tid = threadIdx.x;
   V v[MAXB][R];
-  int base[MAXB], jj[MAXB];
+  int qq[MAXB], kk[MAXB];
 #pragma unroll
   for (int it = 0; it < MAXB; ++it) {
-    const int b = tid + it * NT;
-    jj[it] = -1;
-    if (b < total) {
-      int line, j;
-      if (line_fast) {
-        line = b % nlines;
-        j = b / nlines;
-      } else {
-        j = b % nb;
-        line = b / nb;
-      }
-      jj[it] = j;
-      base[it] = line * ls;
-      const int k = j % Ns;
+    const int j = lane + 64 * it;
+    if (j < nb) {
+      const int q = magic ? (int)__umulhi((unsigned)j, magic) : j;
+      const int k = j - (int)__umul24(q, Ns);
+      qq[it] = q;
+      kk[it] = k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        V a = lds[base[it] + (j + r * nb) * ps];
-        if (r > 0 && k > 0) a = cmul(a, tw[k * r * tstep]);
-        v[it][r] = a;
+      for (int r = 0; r < R; ++r) v[it][r] = line[(j + r * nb)];
+      if (Ns > 1) {
+        const int kt = (int)__umul24(k, tstep);
+        int ti = kt;
+#pragma unroll
+        for (int r = 1; r < R; ++r, ti += kt) v[it][r] = cmul(v[it][r], tw[ti]);
       }
     }
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int it = 0; it < MAXB; ++it) {
-    if (jj[it] >= 0) {
+    const int j = lane + 64 * it;
+    if (j < nb) {
       Dft<T, R>::run(v[it]);
-      const int j = jj[it];
-      const int k = j % Ns;
-      const int d0 = (j / Ns) * Ns * R + k;
+      const int d0 = (int)__umul24(qq[it], Ns * R) + kk[it];
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[base[it] + (d0 + r * Ns) * ps] = v[it][r];
+      for (int r = 0; r < R; ++r) line[(d0 + r * Ns)] = v[it][r];
     }
   }
-  __syncthreads();
+  wave_sync();
 }
 
 template <typename T>
-__device__ void fft_lines(typename C2<T>::t *lds, int nlines, const AxisPlan &ax, int ls, int ps,
-                          const typename C2<T>::t *tw, bool line_fast) {
+__device__ void wave_fft(typename C2<T>::t *line, const AxisPlan &ax, const typename C2<T>::t *tw,
+                         int lane) {
   int Ns = 1;
   for (int s = 0; s < ax.nst; ++s) {
     const int R = ax.rad[s];
     switch (R) {
-      case 2: fft_stage<T, 2>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
-      case 3: fft_stage<T, 3>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
-      case 4: fft_stage<T, 4>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
-      case 5: fft_stage<T, 5>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
-      default: fft_stage<T, 8>(lds, nlines, ax.n, Ns, ls, ps, tw, line_fast); break;
+      case 2: wave_stage<T, 2>(line, ax.n, Ns, ax.magic[s], tw, lane); break;
+      case 3: wave_stage<T, 3>(line, ax.n, Ns, ax.magic[s], tw, lane); break;
+      case 4: wave_stage<T, 4>(line, ax.n, Ns, ax.magic[s], tw, lane); break;
+      case 5: wave_stage<T, 5>(line, ax.n, Ns, ax.magic[s], tw, lane); break;
+      default: wave_stage<T, 8>(line, ax.n, Ns, ax.magic[s], tw, lane); break;
     }
     Ns *= R;
   }
@@ -246,11 +242,37 @@ __device__ __forceinline__ int overlap(int i0, int ns, int n, int t0, int tw, in
   return 0;
 }
 
+// Copy total elements with U loads in flight per thread: load(idx) reads
+// global memory, store(idx, v) writes LDS.  Loading every element before
+// storing any keeps the loads of a thread independent of each other.
+template <int U, typename V, typename Load, typename Store>
+__device__ __forceinline__ void batched_copy(int total, Load load, Store store) {
+  for (int base = 0; base < total; base += U * NT) {
+    V tmp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = base + u * NT + (int)threadIdx.x;
+      if (idx < total) tmp[u] = load(idx);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = base + u * NT + (int)threadIdx.x;
+      if (idx < total) store(idx, tmp[u]);
+    }
+  }
+}
+
+// q = c / d for small non-negative c, d through a float reciprocal; the 0.5
+// offset keeps c / d at least 1 / (2 d) away from an integer.
+__device__ __forceinline__ int small_div(int c, float inv_d) { return (int)(((float)c + 0.5f) * inv_d); }
+
 // ---------------------------------------------------------------------------
 // Atom preparation and counting sort
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__restrict__ species,
-                       double *__restrict__ X, int32_t *__restrict__ key,
+                       double *__restrict__ X, int4 *__restrict__ i0o, T *__restrict__ phi,
+                       T *__restrict__ dphi, int32_t *__restrict__ key,
                        int32_t *__restrict__ count) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= g.natoms) return;
@@ -258,6 +280,8 @@ __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__r
   const double d1 = pos[3 * j + 1] - g.origin[1];
   const double d2 = pos[3 * j + 2] - g.origin[2];
   double x[3];
+  int i0[3];
+  const int ns = g.ns;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     double s = g.ainv[i] * d0 + g.ainv[i + 3] * d1 + g.ainv[i + 6] * d2;
@@ -266,7 +290,16 @@ __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__r
     if (u >= (double)g.n[i]) u -= (double)g.n[i];
     x[i] = u;
     X[3 * j + i] = u;
+    i0[i] = (int)ceil(u - g.half_ns);
+    T *ph = phi + (size_t)j * 3 * ns + i * ns;
+    T *dp = dphi ? dphi + (size_t)j * 3 * ns + i * ns : nullptr;
+    for (int k = 0; k < ns; ++k) {
+      T d;
+      ph[k] = es_phi<T>((double)(i0[i] + k) - u, (double)ns, g.beta, dp ? &d : (T *)nullptr);
+      if (dp) dp[k] = d;
+    }
   }
+  i0o[j] = int4{i0[0], i0[1], i0[2], 0};
   const int tx = min((int)x[0] / g.txr, g.ntx - 1);
   const int ty = min((int)x[1] / g.ty, g.nty - 1);
   const int f = (g.F == 1) ? 0 : species[j];
@@ -309,10 +342,26 @@ __global__ void k_scatter(int64_t natoms, const int32_t *__restrict__ key, int32
   sidx[p] = (int32_t)j;
 }
 
+// Order every bin by atom index: one wave per bin; bins of up to 64 atoms
+// are ranked in registers, larger ones fall back to an insertion sort.
 __global__ void k_binsort(int nb, const int32_t *__restrict__ start, int32_t *__restrict__ sidx) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= nb) return;
   const int k0 = start[b], k1 = start[b + 1];
+  const int n = k1 - k0;
+  if (n <= 1) return;
+  if (n <= 64) {
+    const int v = lane < n ? sidx[k0 + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      const int w = __shfl(v, i);
+      rank += (w < v) ? 1 : 0;
+    }
+    if (lane < n) sidx[k0 + rank] = v;
+    return;
+  }
+  if (lane != 0) return;
   for (int k = k0 + 1; k < k1; ++k) {
     const int v = sidx[k];
     int i = k - 1;
@@ -336,15 +385,140 @@ __device__ __forceinline__ void bin_window(int t, int h, int nt, int *first, int
 }
 
 // ---------------------------------------------------------------------------
+// Candidate staging shared by the spreader and the gather.  The atoms whose
+// stencil reaches the tile are compacted into LDS in bin order (a block scan
+// of the overlap flags), so every round is deterministic.
+//   rec.x = atom, rec.y = kx | xl << 8 | ox << 16, rec.z = ky | yl << 8 | oy << 16,
+//   rec.w = first z point of the stencil, wrapped into [0, n3)
+// ---------------------------------------------------------------------------
+constexpr int kMaxWindowBins = 128;
+
+struct Stage {
+  int nbin;                           // bins in the window
+  int total;                          // candidate atoms in the window
+  int bin_first[kMaxWindowBins];      // sorted index of each bin's first atom
+  int bin_pref[kMaxWindowBins + 1];   // candidates before each bin
+  int wsum[NW];
+  int nstaged;
+  int slab_n[NW];            // staged atoms that reach each wave's z slab
+  int slab_w[NW][NW];        // per-wave counts of one chunk, by slab
+};
+
+__device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
+                             const int32_t *__restrict__ start) {
+  int bxs, bxn, bys, byn;
+  bin_window(tx, g.hx, g.ntx, &bxs, &bxn);
+  bin_window(ty, g.hy, g.nty, &bys, &byn);
+  const int nb = bxn * byn;
+  const int t = threadIdx.x;
+  if (t < nb) {
+    const int by = wrapi(bys + t / bxn, g.nty);
+    const int bx = wrapi(bxs + t % bxn, g.ntx);
+    const int bin = (f * g.nty + by) * g.ntx + bx;
+    const int s0 = start[bin];
+    st.bin_first[t] = s0;
+    st.bin_pref[t + 1] = start[bin + 1] - s0;  // counts, scanned below
+  }
+  __syncthreads();
+  if (t == 0) {
+    int tot = 0;
+    st.bin_pref[0] = 0;
+    for (int b = 0; b < nb; ++b) {
+      tot += st.bin_pref[b + 1];
+      st.bin_pref[b + 1] = tot;
+    }
+    st.nbin = nb;
+    st.total = tot;
+    st.nstaged = 0;
+  }
+  __syncthreads();
+}
+
+// Stage candidates [base, base + NT) of the window behind the ones already
+// staged; returns (block-uniform) the number staged so far.
+__device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0, int txw, int y0,
+                           int tyw, const int32_t *__restrict__ sidx,
+                           const int4 *__restrict__ i0a) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int t = base + tid;
+  int flag = 0;
+  int4 r = int4{0, 0, 0, 0};
+  if (t < st.total) {
+    int b = 0;
+    while (st.bin_pref[b + 1] <= t) ++b;
+    const int j = sidx[st.bin_first[b] + (t - st.bin_pref[b])];
+    const int4 i0 = i0a[j];
+    int kx = 0, xl = 0, ky = 0, yl = 0;
+    const int ox = overlap(i0.x, g.ns, g.n[0], x0, txw, &kx, &xl);
+    const int oy = ox ? overlap(i0.y, g.ns, g.n[1], y0, tyw, &ky, &yl) : 0;
+    if (ox && oy) {
+      flag = 1;
+      r = int4{j, kx | (xl << 8) | (ox << 16), ky | (yl << 8) | (oy << 16), wrapi(i0.z, g.n[2])};
+    }
+  }
+  const unsigned long long m = __ballot(flag);
+  const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+  if (lane == 0) st.wsum[wave] = __popcll(m);
+  __syncthreads();
+  int off = st.nstaged;
+  for (int w = 0; w < wave; ++w) off += st.wsum[w];
+  if (flag) rec[off + before] = r;
+  int total = st.nstaged;
+  for (int w = 0; w < NW; ++w) total += st.wsum[w];
+  __syncthreads();
+  if (tid == 0) st.nstaged = total;
+  return total;
+}
+
+// After stage_chunk: append the atoms of this chunk to the per-slab index
+// lists (slab w = z points [w n3 / NW, (w+1) n3 / NW)), in staging order.
+__device__ void slab_lists(Stage &st, const int4 *rec, unsigned short *slist, int first, int last,
+                           int n3, int ns) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the staged atoms of this chunk are rec[first .. last); one per thread
+  const int s = first + tid;
+  unsigned mask = 0;
+  if (s < last) {
+    const int a = rec[s].w;
+    for (int w = 0; w < NW; ++w) {
+      const int z0 = (n3 * w) / NW, z1 = (n3 * (w + 1)) / NW;
+      const bool hit = (max(a, z0) < min(a + ns, z1)) || (a + ns > n3 && z0 < a + ns - n3);
+      if (hit) mask |= 1u << w;
+    }
+  }
+  unsigned long long m[NW];
+  for (int w = 0; w < NW; ++w) {
+    m[w] = __ballot((mask >> w) & 1u);
+    if (lane == 0) st.slab_w[wave][w] = __popcll(m[w]);
+  }
+  __syncthreads();
+  for (int w = 0; w < NW; ++w) {
+    if (!((mask >> w) & 1u)) continue;
+    int off = st.slab_n[w];
+    for (int v = 0; v < wave; ++v) off += st.slab_w[v][w];
+    off += __popcll(m[w] & ((1ULL << lane) - 1ULL));
+    slist[w * kStageCap + off] = (unsigned short)s;
+  }
+  __syncthreads();
+  if (tid < NW) {
+    int add = 0;
+    for (int v = 0; v < NW; ++v) add += st.slab_w[v][tid];
+    st.slab_n[tid] += add;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
 // Spread + z-DFT.  One workgroup per (x,y) tile of txr x ty real columns and
 // one field.  The tile is held in LDS as txr/2 x ty complex columns of n3
-// points: real column 2p in .x, 2p+1 in .y, so one complex DFT transforms two
-// real columns.  Wave w owns the z slab [w n3/4, (w+1) n3/4) and walks the
-// candidate atoms in sorted order, so every LDS cell is updated by one wave,
-// in atom order: no atomics, and the grid is bit-reproducible.
+// points (stride cs): real column 2p in .x, 2p+1 in .y, so one complex DFT
+// transforms two real columns.  Wave w owns the z slab [w n3/4, (w+1) n3/4)
+// and walks the staged atoms in order, so every LDS cell is updated by one
+// wave, in atom order: no atomics, and the grid is bit-reproducible.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const double *__restrict__ X,
+__global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 *__restrict__ i0a,
+                                                 const T *__restrict__ phi,
                                                  const int32_t *__restrict__ sidx,
                                                  const int32_t *__restrict__ start,
                                                  const int32_t *__restrict__ species,
@@ -352,91 +526,88 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const doubl
                                                  typename C2<T>::t *__restrict__ W1) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+  __shared__ Stage st;
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], cs = g.cs;
   const int ccx = g.txr / 2, cc = ccx * g.ty;
   V *tile = reinterpret_cast<V *>(smem);
-  V *tw = tile + (size_t)n3 * cc;
+  V *tw = tile + (size_t)cc * cs;
+  int4 *rec = reinterpret_cast<int4 *>(tw + n3);
+  unsigned short *slist = reinterpret_cast<unsigned short *>(rec + kStageCap);
   T *tflat = reinterpret_cast<T *>(tile);
   const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx, f = blockIdx.y;
   const int x0 = tx * g.txr, y0 = ty * g.ty;
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n3 * cc; i += NT) tile[i] = V{T(0), T(0)};
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
   for (int i = tid; i < n3; i += NT) tw[i] = twz[i];
-  __syncthreads();
+  if (tid < NW) st.slab_n[tid] = 0;
+  stage_window(st, g, tx, ty, f, start);
 
-  const int wave = tid >> 6, lane = tid & 63, nw = NT / 64;
-  const int zs0 = (n3 * wave) / nw, zs1 = (n3 * (wave + 1)) / nw;
+  const int zs0 = (n3 * wave) / NW, zs1 = (n3 * (wave + 1)) / NW;
   const int ns = g.ns;
-  int bxs, bxn, bys, byn;
-  bin_window(tx, g.hx, g.ntx, &bxs, &bxn);
-  bin_window(ty, g.hy, g.nty, &bys, &byn);
-  for (int iby = 0; iby < byn; ++iby) {
-    const int by = wrapi(bys + iby, g.nty);
-    for (int ibx = 0; ibx < bxn; ++ibx) {
-      const int bx = wrapi(bxs + ibx, g.ntx);
-      const int bin = (f * g.nty + by) * g.ntx + bx;
-      const int k0 = start[bin], k1 = start[bin + 1];
-      for (int k = k0; k < k1; ++k) {
-        const int j = sidx[k];
-        const double Xx = X[3 * j], Xy = X[3 * j + 1], Xz = X[3 * j + 2];
-        const int i0x = (int)ceil(Xx - g.half_ns);
-        const int i0y = (int)ceil(Xy - g.half_ns);
-        const int i0z = (int)ceil(Xz - g.half_ns);
-        int kxl, xll, kyl, yll;
-        const int ox = overlap(i0x, ns, n1, x0, txw, &kxl, &xll);
-        if (ox == 0) continue;
-        const int oy = overlap(i0y, ns, n2, y0, tyw, &kyl, &yll);
-        if (oy == 0) continue;
-        // z pieces of the stencil inside this wave's slab
-        const int a = wrapi(i0z, n3);
-        int pk[2], pz[2], pc[2], np = 0;
-        {
-          const int e1 = min(a + ns, n3);
-          const int lo = max(a, zs0), hi = min(e1, zs1);
-          if (lo < hi) { pk[np] = lo - a; pz[np] = lo; pc[np] = hi - lo; ++np; }
-          if (a + ns > n3) {
-            const int e2 = a + ns - n3;
-            const int lo2 = max(0, zs0), hi2 = min(e2, zs1);
-            if (lo2 < hi2) { pk[np] = n3 - a + lo2; pz[np] = lo2; pc[np] = hi2 - lo2; ++np; }
-          }
+  for (int base = 0; base < st.total; base += NT) {
+    const int before = st.nstaged;
+    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a);
+    slab_lists(st, rec, slist, before, nst, n3, ns);
+    const bool last = base + NT >= st.total;
+    if (!last && nst <= kStageCap - NT) continue;
+    // spread the atoms of this wave's slab, in staging order
+    const unsigned short *mine = slist + wave * kStageCap;
+    const int cnt = st.slab_n[wave];
+    T ph_next = (cnt > 0 && lane < 3 * ns) ? phi[(size_t)rec[mine[0]].x * 3 * ns + lane] : T(0);
+    for (int t = 0; t < cnt; ++t) {
+      const int4 r = rec[mine[t]];
+      const int j = r.x;
+      // the kernel values of this atom were loaded one iteration ahead
+      const T ph = ph_next;
+      if (t + 1 < cnt && lane < 3 * ns) ph_next = phi[(size_t)rec[mine[t + 1]].x * 3 * ns + lane];
+      const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
+      const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
+      const int a = r.w;
+      int pk[2], pz[2], pc[2], np = 0;
+      {
+        const int e1 = min(a + ns, n3);
+        const int lo = max(a, zs0), hi = min(e1, zs1);
+        if (lo < hi) { pk[np] = lo - a; pz[np] = lo; pc[np] = hi - lo; ++np; }
+        if (a + ns > n3) {
+          const int e2 = a + ns - n3;
+          const int lo2 = max(0, zs0), hi2 = min(e2, zs1);
+          if (lo2 < hi2) { pk[np] = n3 - a + lo2; pz[np] = lo2; pc[np] = hi2 - lo2; ++np; }
         }
-        if (np == 0) continue;
-        const T w = (g.F == 1) ? (T)g.amp[species[j]] : T(1);
-        T phi = T(0);
-        if (lane < 3 * ns) {
-          const int ax = lane / ns, kk = lane - ax * ns;
-          const double X3 = ax == 0 ? Xx : (ax == 1 ? Xy : Xz);
-          const int i03 = ax == 0 ? i0x : (ax == 1 ? i0y : i0z);
-          phi = es_phi<T>((double)(i03 + kk) - X3, (double)ns, g.beta, (T *)nullptr);
-        }
-        for (int p = 0; p < np; ++p) {
-          const int cells = ox * oy * pc[p];
-          const int iters = (cells + 63) / 64;
-          for (int it = 0; it < iters; ++it) {
-            // every lane takes part in the shuffles; only c < cells writes
-            const int c = lane + 64 * it;
-            const int cv = c < cells ? c : 0;
-            const int ix = cv % ox;
-            const int t2 = cv / ox;
-            const int iy = t2 % oy;
-            const int kz = t2 / oy;
-            const T fx = __shfl(phi, kxl + ix);
-            const T fy = __shfl(phi, ns + kyl + iy);
-            const T fz = __shfl(phi, 2 * ns + pk[p] + kz);
-            if (c < cells) {
-              const int xl = xll + ix, yl = yll + iy, z = pz[p] + kz;
-              const int col = yl * ccx + (xl >> 1);
-              T *cell = tflat + 2 * ((size_t)z * cc + col) + (xl & 1);
-              *cell += w * fx * fy * fz;
-            }
+      }
+      const T w = (g.F == 1) ? (T)g.amp[species[j]] : T(1);
+      const float inv_ox = 1.0f / (float)ox;
+      for (int p = 0; p < np; ++p) {
+        const int cells = ox * oy * pc[p];
+        const float inv_pc = 1.0f / (float)pc[p];
+        for (int c0 = 0; c0 < cells; c0 += 64) {
+          // every lane takes part in the shuffles; only c < cells writes
+          const int c = c0 + lane;
+          const int cv = c < cells ? c : 0;
+          const int t1 = small_div(cv, inv_pc);
+          const int kz = cv - t1 * pc[p];
+          const int iy = small_div(t1, inv_ox);
+          const int ix = t1 - iy * ox;
+          const T fz = __shfl(ph, 2 * ns + pk[p] + kz);
+          const T fx = __shfl(ph, kxl + ix);
+          const T fy = __shfl(ph, ns + kyl + iy);
+          if (c < cells) {
+            const int xl = xll + ix, yl = yll + iy, z = pz[p] + kz;
+            const int col = yl * ccx + (xl >> 1);
+            T *cell = tflat + 2 * ((size_t)col * cs + (z)) + (xl & 1);
+            *cell += w * fx * fy * fz;
           }
         }
       }
     }
+    __syncthreads();
+    if (tid == 0) st.nstaged = 0;
+    if (tid < NW) st.slab_n[tid] = 0;
+    __syncthreads();
   }
   __syncthreads();
-  fft_lines<T>(tile, cc, az, 1, cc, tw, true);
+  for (int col = wave; col < cc; col += NW) wave_fft<T>(tile + (size_t)col * cs, az, tw, lane);
+  __syncthreads();
   // split the packed pairs and write h3 = 0..M3
   const int M3 = g.M[2];
   const int per = tyw * txw;
@@ -446,8 +617,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const doubl
     const int yl = r % tyw;
     const int k3 = r / tyw;
     const int col = yl * ccx + (xl >> 1);
-    const V Z = tile[(size_t)k3 * cc + col];
-    const V Zm = tile[(size_t)((n3 - k3) % n3) * cc + col];
+    const V Z = tile[(size_t)col * cs + (k3)];
+    const V Zm = tile[(size_t)col * cs + ((n3 - k3) % n3)];
     V o;
     if ((xl & 1) == 0)
       o = V{T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y - Zm.y)};
@@ -458,7 +629,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// y-DFT.  One workgroup per (x-chunk, h3, field).
+// y-DFT.  One workgroup per (x-chunk, h3, field); lines are transposed into
+// contiguous LDS rows (stride n2 + 1) and transformed one per wave.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
@@ -470,22 +642,25 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
                                               typename C2<T>::t *__restrict__ W2) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2];
-  V *cols = reinterpret_cast<V *>(smem);
-  V *tw = cols + (size_t)n2 * tx2;
+  const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
+  V *lines = reinterpret_cast<V *>(smem);
+  V *tw = lines + (size_t)tx2 * ls;
   const int x0 = blockIdx.x * tx2, k3 = blockIdx.y, f = blockIdx.z;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lo = lo2[k3], hi = hi2[k3];
   if (hi < lo) return;
   for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
   const V *src = W1 + ((size_t)f * (M3 + 1) + k3) * n2 * n1;
-  for (int idx = tid; idx < n2 * tx2; idx += NT) {
-    const int xl = idx % tx2, y = idx / tx2;
-    const int x = x0 + xl;
-    cols[idx] = x < n1 ? src[(size_t)y * n1 + x] : V{T(0), T(0)};
-  }
+  batched_copy<8, V>(
+      n2 * tx2,
+      [&](int idx) {
+        const int x = x0 + idx % tx2, y = idx / tx2;
+        return x < n1 ? src[(size_t)y * n1 + x] : V{T(0), T(0)};
+      },
+      [&](int idx, V v) { lines[(idx % tx2) * ls + (idx / tx2)] = v; });
   __syncthreads();
-  fft_lines<T>(cols, tx2, ay, 1, tx2, tw, true);
+  for (int l = wave; l < tx2; l += NW) wave_fft<T>(lines + (size_t)l * ls, ay, tw, lane);
+  __syncthreads();
   const int nl = hi - lo + 1;
   V *dst = W2 + ((size_t)f * g.nlines + loff[k3]) * n1;
   for (int idx = tid; idx < nl * tx2; idx += NT) {
@@ -494,7 +669,7 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
     if (x >= n1) continue;
     const int h2 = lo + r;
     const int yi = h2 < 0 ? h2 + n2 : h2;
-    dst[(size_t)r * n1 + x] = cols[yi * tx2 + xl];
+    dst[(size_t)r * n1 + x] = lines[xl * ls + (yi)];
   }
 }
 
@@ -512,6 +687,37 @@ __device__ __forceinline__ double xray_f(const double *c, double q) {
   return f;
 }
 
+__device__ __forceinline__ double qlen_of(const Geo &g, int h1, int h2, int h3) {
+  const double h1d = h1, h2d = h2, h3d = h3;
+  const double qx = g.b[0] * h1d + g.b[3] * h2d + g.b[6] * h3d;
+  const double qy = g.b[1] * h1d + g.b[4] * h2d + g.b[7] * h3d;
+  const double qz = g.b[2] * h1d + g.b[5] * h2d + g.b[8] * h3d;
+  return sqrt(qx * qx + qy * qy + qz * qz);
+}
+
+// Per-group line tables staged in LDS by the x-pass kernels.
+struct LineGroup {
+  int off[kMaxLb + 1];     // candidate offset of each line (and the end)
+  int h[kMaxLb][3];        // h2, h3, lo1
+  double d23[kMaxLb];      // 1 / (phihat_2(|h2|) phihat_3(h3))
+};
+
+__device__ __forceinline__ void load_group(LineGroup &lg, int l0, int nl,
+                                           const int4 *__restrict__ line_info,
+                                           const int32_t *__restrict__ line_off,
+                                           const double *__restrict__ ph2,
+                                           const double *__restrict__ ph3) {
+  const int tid = threadIdx.x;
+  if (tid <= nl) lg.off[tid] = line_off[l0 + tid];
+  if (tid < nl) {
+    const int4 info = line_info[l0 + tid];
+    lg.h[tid][0] = info.x;
+    lg.h[tid][1] = info.y;
+    lg.h[tid][2] = info.z;
+    lg.d23[tid] = 1.0 / (ph2[abs(info.x)] * ph3[info.y]);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, int ngroups,
                                                   const typename C2<T>::t *__restrict__ W2,
@@ -527,39 +733,46 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
                                                   int64_t ncand, double *__restrict__ gnum) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n1 = g.n[0], F = g.F, nq = g.nq, ncomp = g.ncomp;
+  __shared__ LineGroup lg;
+  const int n1 = g.n[0], F = g.F, nq = g.nq, ncomp = g.ncomp, M1 = g.M[0];
+  const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
-  V *tw = rows + (size_t)lb * F * n1;
+  V *tw = rows + (size_t)lb * F * rs;
   double *acc = reinterpret_cast<double *>(tw + n1);
-  const int tid = threadIdx.x;
+  double *inv1 = acc + (size_t)ncomp * nq;  // 1 / phihat_1(|h1|)
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
+  for (int i = tid; i <= M1; i += NT) inv1[i] = 1.0 / ph1[i];
   const int S = g.S;
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int l0 = grp * lb;
     const int nl = min(lb, g.nlines - l0);
     __syncthreads();
-    for (int idx = tid; idx < nl * F * n1; idx += NT) {
-      const int x = idx % n1, r = idx / n1;
-      const int li = r / F, f = r % F;
-      rows[idx] = W2[((size_t)f * g.nlines + l0 + li) * n1 + x];
-    }
+    load_group(lg, l0, nl, line_info, line_off, ph2, ph3);
+    batched_copy<8, V>(
+        nl * F * n1,
+        [&](int idx) {
+          const int x = idx % n1, r = idx / n1;
+          return W2[((size_t)(r % F) * g.nlines + l0 + r / F) * n1 + x];
+        },
+        [&](int idx, V v) { rows[(idx / n1) * rs + (idx % n1)] = v; });
     __syncthreads();
-    fft_lines<T>(rows, nl * F, ax1, n1, 1, tw, false);
-    const int c0 = line_off[l0], c1 = line_off[l0 + nl];
+    for (int r = wave; r < nl * F; r += NW) wave_fft<T>(rows + (size_t)r * rs, ax1, tw, lane);
+    __syncthreads();
+    const int c0 = lg.off[0], c1 = lg.off[nl];
     for (int c = c0 + tid; c < c1; c += NT) {
       const unsigned s = stab[c];
       if (s == kNoShell) continue;
       int li = 0;
-      while (li + 1 < nl && line_off[l0 + li + 1] <= c) ++li;
-      const int4 info = line_info[l0 + li];
-      const int h2 = info.x, h3 = info.y, lo1 = info.z;
-      const int h1 = lo1 + (c - line_off[l0 + li]);
-      const double d = 1.0 / (ph1[abs(h1)] * ph2[abs(h2)] * ph3[h3]);
+      while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
+      const int h2 = lg.h[li][0], h3 = lg.h[li][1];
+      const int h1 = lg.h[li][2] + (c - lg.off[li]);
+      const double d = lg.d23[li] * inv1[abs(h1)];
       const int ix = h1 < 0 ? h1 + n1 : h1;
       double rr[kMaxSpecies], ri[kMaxSpecies];
       for (int f = 0; f < F; ++f) {
-        const V v = rows[((size_t)li * F + f) * n1 + ix];
+        const V v = rows[((size_t)li * F + f) * rs + (ix)];
         rr[f] = (double)v.x * d;
         ri[f] = (double)v.y * d;
         if (rho_out) rho_out[(size_t)f * ncand + c] = V{(T)rr[f], (T)ri[f]};
@@ -569,14 +782,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
         tr = rr[0];
         ti = ri[0];
       } else {
-        double q = 0.0;
-        if (g.weight_kind == 2) {
-          const double h1d = h1, h2d = h2, h3d = h3;
-          const double qx = g.b[0] * h1d + g.b[3] * h2d + g.b[6] * h3d;
-          const double qy = g.b[1] * h1d + g.b[4] * h2d + g.b[7] * h3d;
-          const double qz = g.b[2] * h1d + g.b[5] * h2d + g.b[8] * h3d;
-          q = sqrt(qx * qx + qy * qy + qz * qz);
-        }
+        const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
         tr = 0.0;
         ti = 0.0;
         for (int a = 0; a < S; ++a) {
@@ -601,26 +807,53 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
   for (int i = tid; i < ncomp * nq; i += NT) out[i] = acc[i];
 }
 
-// Ordered, compensated (Neumaier) sum over the workgroup partials.
-__global__ void k_finalize(const double *__restrict__ wgpart, int nblk, int ntot,
-                           double *__restrict__ acc, double *__restrict__ frame_out, int accumulate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ntot) return;
+// Ordered, compensated (Neumaier) sum over the workgroup partials: each entry
+// gets kFinParts strided partial sums that are combined in a fixed order.
+constexpr int kFinParts = 16;
+constexpr int kFinEntries = 256 / kFinParts;
+
+__device__ __forceinline__ void neumaier(double &s, double &c, double v) {
+  const double t = s + v;
+  c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+  s = t;
+}
+
+__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ wgpart, int nblk,
+                                                  int ntot, double *__restrict__ acc,
+                                                  double *__restrict__ frame_out, int accumulate) {
+  __shared__ double ps[kFinParts][kFinEntries], pc[kFinParts][kFinEntries];
+  const int e = threadIdx.x % kFinEntries, part = threadIdx.x / kFinEntries;
+  const int i = blockIdx.x * kFinEntries + e;
   double s = 0.0, c = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    const double v = wgpart[(size_t)b * ntot + i];
-    const double t = s + v;
-    c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
-    s = t;
+  if (i < ntot) {
+    int b = part;
+    for (; b + 3 * kFinParts < nblk; b += 4 * kFinParts) {
+      const double v0 = wgpart[(size_t)b * ntot + i];
+      const double v1 = wgpart[(size_t)(b + kFinParts) * ntot + i];
+      const double v2 = wgpart[(size_t)(b + 2 * kFinParts) * ntot + i];
+      const double v3 = wgpart[(size_t)(b + 3 * kFinParts) * ntot + i];
+      neumaier(s, c, v0);
+      neumaier(s, c, v1);
+      neumaier(s, c, v2);
+      neumaier(s, c, v3);
+    }
+    for (; b < nblk; b += kFinParts) neumaier(s, c, wgpart[(size_t)b * ntot + i]);
   }
-  const double total = s + c;
+  ps[part][e] = s;
+  pc[part][e] = c;
+  __syncthreads();
+  if (part != 0 || i >= ntot) return;
+  double ts = 0.0, tc = 0.0;
+  for (int k = 0; k < kFinParts; ++k) {
+    neumaier(ts, tc, ps[k][e]);
+    tc += pc[k][e];
+  }
+  const double total = ts + tc;
   if (frame_out) frame_out[i] = total;
   if (accumulate) {
-    const double a = acc[i];
-    double cc = acc[ntot + i];
-    const double t = a + total;
-    cc += (fabs(a) >= fabs(total)) ? (a - t) + total : (total - t) + a;
-    acc[i] = t;
+    double a = acc[i], cc = acc[ntot + i];
+    neumaier(a, cc, total);
+    acc[i] = a;
     acc[ntot + i] = cc;
   }
 }
@@ -648,26 +881,30 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n1 = g.n[0], F = g.F, nq = g.nq, S = g.S;
+  __shared__ LineGroup lg;
+  const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
-  V *tw = rows + (size_t)lb * Fa * n1;
-  const int tid = threadIdx.x;
+  V *tw = rows + (size_t)lb * Fa * rs;
+  double *inv1 = reinterpret_cast<double *>(tw + n1);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
+  for (int i = tid; i <= g.M[0]; i += NT) inv1[i] = 1.0 / ph1[i];
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int l0 = grp * lb;
     const int nl = min(lb, g.nlines - l0);
     __syncthreads();
-    for (int idx = tid; idx < nl * Fa * n1; idx += NT) rows[idx] = V{T(0), T(0)};
+    load_group(lg, l0, nl, line_info, line_off, ph2, ph3);
+    for (int idx = tid; idx < nl * Fa * rs; idx += NT) rows[idx] = V{T(0), T(0)};
     __syncthreads();
-    const int c0 = line_off[l0], c1 = line_off[l0 + nl];
+    const int c0 = lg.off[0], c1 = lg.off[nl];
     for (int c = c0 + tid; c < c1; c += NT) {
       const unsigned s = stab[c];
       if (s == kNoShell) continue;
       int li = 0;
-      while (li + 1 < nl && line_off[l0 + li + 1] <= c) ++li;
-      const int4 info = line_info[l0 + li];
-      const int h2 = info.x, h3 = info.y, lo1 = info.z;
-      const int h1 = lo1 + (c - line_off[l0 + li]);
-      const double d = 1.0 / (ph1[abs(h1)] * ph2[abs(h2)] * ph3[h3]);
+      while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
+      const int h2 = lg.h[li][0], h3 = lg.h[li][1];
+      const int h1 = lg.h[li][2] + (c - lg.off[li]);
+      const double d = lg.d23[li] * inv1[abs(h1)];
       const int ix = h1 < 0 ? h1 + n1 : h1;
       double rr[kMaxSpecies], ri[kMaxSpecies], w[kMaxSpecies];
       for (int f = 0; f < F; ++f) {
@@ -675,19 +912,12 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
         rr[f] = v.x;
         ri[f] = v.y;
       }
-      double q = 0.0;
-      if (g.weight_kind == 2) {
-        const double h1d = h1, h2d = h2, h3d = h3;
-        const double qx = g.b[0] * h1d + g.b[3] * h2d + g.b[6] * h3d;
-        const double qy = g.b[1] * h1d + g.b[4] * h2d + g.b[7] * h3d;
-        const double qz = g.b[2] * h1d + g.b[5] * h2d + g.b[8] * h3d;
-        q = sqrt(qx * qx + qy * qy + qz * qz);
-      }
       double tr = 0.0, ti = 0.0;
       if (F == 1) {
         tr = rr[0];
         ti = ri[0];
       } else {
+        const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
         for (int a = 0; a < S; ++a) {
           w[a] = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
           tr += w[a] * rr[a];
@@ -707,15 +937,16 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
               if (cf == b) { cr += H * rr[a]; ci -= H * ri[a]; }
             }
         }
-        rows[((size_t)li * Fa + cf) * n1 + ix] = V{(T)(cr * d), (T)(ci * d)};
+        rows[((size_t)li * Fa + cf) * rs + (ix)] = V{(T)(cr * d), (T)(ci * d)};
       }
     }
     __syncthreads();
-    fft_lines<T>(rows, nl * Fa, ax1, n1, 1, tw, false);
+    for (int r = wave; r < nl * Fa; r += NW) wave_fft<T>(rows + (size_t)r * rs, ax1, tw, lane);
+    __syncthreads();
     for (int idx = tid; idx < nl * Fa * n1; idx += NT) {
       const int x = idx % n1, r = idx / n1;
       const int li = r / Fa, cf = r % Fa;
-      W2[((size_t)cf * g.nlines + l0 + li) * n1 + x] = rows[idx];
+      W2[((size_t)cf * g.nlines + l0 + li) * n1 + x] = rows[r * rs + (x)];
     }
   }
 }
@@ -730,32 +961,36 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
                                               typename C2<T>::t *__restrict__ W1) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2];
-  V *cols = reinterpret_cast<V *>(smem);
-  V *tw = cols + (size_t)n2 * tx2;
+  const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
+  V *lines = reinterpret_cast<V *>(smem);
+  V *tw = lines + (size_t)tx2 * ls;
   const int x0 = blockIdx.x * tx2, k3 = blockIdx.y, cf = blockIdx.z;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lo = lo2[k3], hi = hi2[k3];
   for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
-  for (int i = tid; i < n2 * tx2; i += NT) cols[i] = V{T(0), T(0)};
+  for (int i = tid; i < tx2 * ls; i += NT) lines[i] = V{T(0), T(0)};
   __syncthreads();
   const V *src = W2 + ((size_t)cf * g.nlines + (hi >= lo ? loff[k3] : 0)) * n1;
-  for (int idx = tid; idx < (hi - lo + 1) * tx2; idx += NT) {
-    const int xl = idx % tx2, r = idx / tx2;
-    const int x = x0 + xl;
-    if (x >= n1) continue;
-    const int h2 = lo + r;
-    const int yi = h2 < 0 ? h2 + n2 : h2;
-    cols[yi * tx2 + xl] = src[(size_t)r * n1 + x];
-  }
+  batched_copy<8, V>(
+      (hi - lo + 1) * tx2,
+      [&](int idx) {
+        const int x = x0 + idx % tx2;
+        return x < n1 ? src[(size_t)(idx / tx2) * n1 + x] : V{T(0), T(0)};
+      },
+      [&](int idx, V v) {
+        const int h2 = lo + idx / tx2;
+        const int yi = h2 < 0 ? h2 + n2 : h2;
+        if (x0 + idx % tx2 < n1) lines[(idx % tx2) * ls + (yi)] = v;
+      });
   __syncthreads();
-  fft_lines<T>(cols, tx2, ay, 1, tx2, tw, true);
+  for (int l = wave; l < tx2; l += NW) wave_fft<T>(lines + (size_t)l * ls, ay, tw, lane);
+  __syncthreads();
   V *dst = W1 + ((size_t)cf * (M3 + 1) + k3) * n2 * n1;
   for (int idx = tid; idx < n2 * tx2; idx += NT) {
     const int xl = idx % tx2, y = idx / tx2;
     const int x = x0 + xl;
     if (x >= n1) continue;
-    V v = cols[idx];
+    V v = lines[xl * ls + (y)];
     // the h3 = 0 plane holds only the canonical half of (h1, h2): the full
     // plane sum is that half plus its conjugate
     if (k3 == 0) v = V{T(2) * v.x, T(0)};
@@ -767,143 +1002,139 @@ template <typename T>
 __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
                                                      const typename C2<T>::t *__restrict__ W1,
                                                      const typename C2<T>::t *__restrict__ twz,
-                                                     const double *__restrict__ X,
+                                                     const int4 *__restrict__ i0a,
+                                                     const T *__restrict__ phi,
+                                                     const T *__restrict__ dphi,
                                                      const int32_t *__restrict__ sidx,
                                                      const int32_t *__restrict__ start,
                                                      double *__restrict__ slots) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2];
+  __shared__ Stage st;
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2], cs = g.cs;
   const int ccx = g.txr / 2, cc = ccx * g.ty;
   V *tile = reinterpret_cast<V *>(smem);
-  V *tw = tile + (size_t)n3 * cc;
+  V *tw = tile + (size_t)cc * cs;
+  int4 *rec = reinterpret_cast<int4 *>(tw + n3);
   const T *tflat = reinterpret_cast<const T *>(tile);
   const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx, cf = blockIdx.y;
   const int x0 = tx * g.txr, y0 = ty * g.ty;
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n3 * cc; i += NT) tile[i] = V{T(0), T(0)};
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
   for (int i = tid; i < n3; i += NT) tw[i] = twz[i];
   __syncthreads();
   const int pw = txw / 2;
   const V *src = W1 + (size_t)cf * (M3 + 1) * n2 * n1;
-  for (int idx = tid; idx < (M3 + 1) * tyw * pw; idx += NT) {
-    const int p = idx % pw;
-    const int r = idx / pw;
-    const int yl = r % tyw;
-    const int k3 = r / tyw;
-    const V A = src[((size_t)k3 * n2 + y0 + yl) * n1 + x0 + 2 * p];
-    const V B = src[((size_t)k3 * n2 + y0 + yl) * n1 + x0 + 2 * p + 1];
-    const int col = yl * ccx + p;
-    tile[(size_t)k3 * cc + col] = V{A.x - B.y, A.y + B.x};
-    if (k3 > 0) tile[(size_t)(n3 - k3) * cc + col] = V{A.x + B.y, B.x - A.y};
-  }
+  struct VV {
+    V a, b;
+  };
+  batched_copy<4, VV>(
+      (M3 + 1) * tyw * pw,
+      [&](int idx) {
+        const int p = idx % pw, r = idx / pw;
+        const V *q = src + ((size_t)(r / tyw) * n2 + y0 + r % tyw) * n1 + x0 + 2 * p;
+        return VV{q[0], q[1]};
+      },
+      [&](int idx, VV v) {
+        const int p = idx % pw, r = idx / pw;
+        const int yl = r % tyw, k3 = r / tyw;
+        const int col = yl * ccx + p;
+        const V A = v.a, B = v.b;
+        tile[(size_t)col * cs + (k3)] = V{A.x - B.y, A.y + B.x};
+        if (k3 > 0) tile[(size_t)col * cs + (n3 - k3)] = V{A.x + B.y, B.x - A.y};
+      });
   __syncthreads();
-  fft_lines<T>(tile, cc, az, 1, cc, tw, true);
+  for (int col = wave; col < cc; col += NW) wave_fft<T>(tile + (size_t)col * cs, az, tw, lane);
+  stage_window(st, g, tx, ty, cf, start);
 
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = NT / 64;
   const int ns = g.ns;
   const int S2 = g.sx * g.sy;
-  int bxs, bxn, bys, byn;
-  bin_window(tx, g.hx, g.ntx, &bxs, &bxn);
-  bin_window(ty, g.hy, g.nty, &bys, &byn);
-  int seen = 0;
-  for (int iby = 0; iby < byn; ++iby) {
-    const int by = wrapi(bys + iby, g.nty);
-    for (int ibx = 0; ibx < bxn; ++ibx) {
-      const int bx = wrapi(bxs + ibx, g.ntx);
-      const int bin = (cf * g.nty + by) * g.ntx + bx;
-      const int k0 = start[bin], k1 = start[bin + 1];
-      for (int k = k0; k < k1; ++k, ++seen) {
-        if (seen % nw != wave) continue;
-        const int j = sidx[k];
-        const double Xx = X[3 * j], Xy = X[3 * j + 1], Xz = X[3 * j + 2];
-        const int i0x = (int)ceil(Xx - g.half_ns);
-        const int i0y = (int)ceil(Xy - g.half_ns);
-        const int i0z = (int)ceil(Xz - g.half_ns);
-        int kxl, xll, kyl, yll;
-        const int ox = overlap(i0x, ns, n1, x0, txw, &kxl, &xll);
-        if (ox == 0) continue;
-        const int oy = overlap(i0y, ns, n2, y0, tyw, &kyl, &yll);
-        if (oy == 0) continue;
-        T phi = T(0), dphi = T(0);
-        if (lane < 3 * ns) {
-          const int ax = lane / ns, kk = lane - ax * ns;
-          const double X3 = ax == 0 ? Xx : (ax == 1 ? Xy : Xz);
-          const int i03 = ax == 0 ? i0x : (ax == 1 ? i0y : i0z);
-          phi = es_phi<T>((double)(i03 + kk) - X3, (double)ns, g.beta, &dphi);
-        }
-        const int a = wrapi(i0z, n3);
-        double gx = 0.0, gy = 0.0, gz = 0.0;
-        const int cells = ox * oy * ns;
-        const int iters = (cells + 63) / 64;
-        for (int it = 0; it < iters; ++it) {
-          const int c = lane + 64 * it;
-          const int cv = c < cells ? c : 0;
-          const int ix = cv % ox;
-          const int t2 = cv / ox;
-          const int iy = t2 % oy;
-          const int kz = t2 / oy;
-          const T fx = __shfl(phi, kxl + ix), dfx = __shfl(dphi, kxl + ix);
-          const T fy = __shfl(phi, ns + kyl + iy), dfy = __shfl(dphi, ns + kyl + iy);
-          const T fz = __shfl(phi, 2 * ns + kz), dfz = __shfl(dphi, 2 * ns + kz);
-          if (c < cells) {
-            const int xl = xll + ix, yl = yll + iy;
-            int z = a + kz;
-            if (z >= n3) z -= n3;
-            const int col = yl * ccx + (xl >> 1);
-            const double v = (double)tflat[2 * ((size_t)z * cc + col) + (xl & 1)];
-            gx += v * (double)(dfx * fy * fz);
-            gy += v * (double)(fx * dfy * fz);
-            gz += v * (double)(fx * fy * dfz);
-          }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          gx += __shfl_xor(gx, off);
-          gy += __shfl_xor(gy, off);
-          gz += __shfl_xor(gz, off);
-        }
-        if (lane == 0) {
-          const int xs = wrapi(i0x, n1), ys = wrapi(i0y, n2);
-          const int dx = wrapi(tx - xs / g.txr, g.ntx);
-          const int dy = wrapi(ty - ys / g.ty, g.nty);
-          double *o = slots + ((size_t)k * S2 + dx * g.sy + dy) * 3;
-          o[0] = gx;
-          o[1] = gy;
-          o[2] = gz;
+  for (int base = 0; base < st.total; base += NT) {
+    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a);
+    const bool last = base + NT >= st.total;
+    if (!last && nst <= kStageCap - NT) continue;
+    for (int s = wave; s < nst; s += NW) {
+      const int4 r = rec[s];
+      const int j = r.x;
+      const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
+      const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
+      const int a = r.w;
+      const T ph = lane < 3 * ns ? phi[(size_t)j * 3 * ns + lane] : T(0);
+      const T dph = lane < 3 * ns ? dphi[(size_t)j * 3 * ns + lane] : T(0);
+      double gx = 0.0, gy = 0.0, gz = 0.0;
+      const int cells = ox * oy * ns;
+      const float inv_ns = 1.0f / (float)ns, inv_ox = 1.0f / (float)ox;
+      for (int c0 = 0; c0 < cells; c0 += 64) {
+        const int c = c0 + lane;
+        const int cv = c < cells ? c : 0;
+        const int t1 = small_div(cv, inv_ns);
+        const int kz = cv - t1 * ns;
+        const int iy = small_div(t1, inv_ox);
+        const int ix = t1 - iy * ox;
+        const T fx = __shfl(ph, kxl + ix), dfx = __shfl(dph, kxl + ix);
+        const T fy = __shfl(ph, ns + kyl + iy), dfy = __shfl(dph, ns + kyl + iy);
+        const T fz = __shfl(ph, 2 * ns + kz), dfz = __shfl(dph, 2 * ns + kz);
+        if (c < cells) {
+          const int xl = xll + ix, yl = yll + iy;
+          int z = a + kz;
+          if (z >= n3) z -= n3;
+          const int col = yl * ccx + (xl >> 1);
+          const double v = (double)tflat[2 * ((size_t)col * cs + (z)) + (xl & 1)];
+          gx += v * (double)(dfx * fy * fz);
+          gy += v * (double)(fx * dfy * fz);
+          gz += v * (double)(fx * fy * dfz);
         }
       }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        gx += __shfl_xor(gx, off);
+        gy += __shfl_xor(gy, off);
+        gz += __shfl_xor(gz, off);
+      }
+      if (lane == 0) {
+        // slot of this (atom, tile) pair: tiles counted from the one that
+        // holds the first stencil point
+        const int4 i0 = i0a[j];
+        const int xs = wrapi(i0.x, n1), ys = wrapi(i0.y, n2);
+        const int dx = wrapi(tx - xs / g.txr, g.ntx);
+        const int dy = wrapi(ty - ys / g.ty, g.nty);
+        double *o = slots + ((size_t)j * S2 + dx * g.sy + dy) * 3;
+        o[0] = gx;
+        o[1] = gy;
+        o[2] = gz;
+      }
     }
+    __syncthreads();
+    if (tid == 0) st.nstaged = 0;
+    __syncthreads();
   }
 }
 
 // Sum each atom's slots in a fixed order and map grid derivatives to dL/dr:
 // dL/dr = sum_i b_i (n_i / 2 pi) dV/dX_i, dV/dX_i = -sum_g V phi'(u_i) prod phi.
-__global__ void k_adj_sum(Geo g, const double *__restrict__ X, const int32_t *__restrict__ sidx,
-                          const int32_t *__restrict__ start, const int32_t *__restrict__ species,
-                          const double *__restrict__ slots, int Fa, double *__restrict__ grad) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.natoms) return;
-  const int j = sidx[k];
-  const double Xx = X[3 * j], Xy = X[3 * j + 1];
-  const int i0x = (int)ceil(Xx - g.half_ns), i0y = (int)ceil(Xy - g.half_ns);
-  const int xs = wrapi(i0x, g.n[0]), ys = wrapi(i0y, g.n[1]);
+__global__ void k_adj_sum(Geo g, const int4 *__restrict__ i0a,
+                          const int32_t *__restrict__ species, const double *__restrict__ slots,
+                          int Fa, double *__restrict__ grad) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.natoms) return;
+  const int4 i0 = i0a[j];
+  const int xs = wrapi(i0.x, g.n[0]), ys = wrapi(i0.y, g.n[1]);
   const int tx0 = xs / g.txr, ty0 = ys / g.ty;
   const int S2 = g.sx * g.sy;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (int dx = 0; dx < g.sx; ++dx) {
     const int tx = wrapi(tx0 + dx, g.ntx);
+    if (dx > 0 && tx == tx0) break;
     const int x0 = tx * g.txr, txw = min(g.txr, g.n[0] - x0);
     int a, b;
-    if (overlap(i0x, g.ns, g.n[0], x0, txw, &a, &b) == 0) continue;
-    if (dx > 0 && tx == tx0) break;
+    if (overlap(i0.x, g.ns, g.n[0], x0, txw, &a, &b) == 0) continue;
     for (int dy = 0; dy < g.sy; ++dy) {
       const int ty = wrapi(ty0 + dy, g.nty);
       if (dy > 0 && ty == ty0) break;
       const int y0 = ty * g.ty, tyw = min(g.ty, g.n[1] - y0);
-      if (overlap(i0y, g.ns, g.n[1], y0, tyw, &a, &b) == 0) continue;
-      const double *o = slots + ((size_t)k * S2 + dx * g.sy + dy) * 3;
+      if (overlap(i0.y, g.ns, g.n[1], y0, tyw, &a, &b) == 0) continue;
+      const double *o = slots + ((size_t)j * S2 + dx * g.sy + dy) * 3;
       s0 += o[0];
       s1 += o[1];
       s2 += o[2];
@@ -965,29 +1196,35 @@ int collect_stage_times(Plan &p) {
   } while (0)
 
 template <typename T>
-static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
-  using V = typename C2<T>::t;
+static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
   Geo &g = p.geo;
   hipStream_t st = p.stream;
   const int nbins_tot = g.F * g.nbins;
   const int blk = 256;
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
-  {
-    StageTimer t(p, 0);
-    hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
-                       p.d_count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
-                       nbins_tot);
-    hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
-                       p.d_cursor, p.d_sidx);
-    hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
-                       p.d_start, p.d_sidx);
-  }
+  StageTimer t(p, 0);
+  hipLaunchKernelGGL(k_prep<T>, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X,
+                     p.d_i0, (T *)p.d_phi, with_dphi ? (T *)p.d_dphi : (T *)nullptr, p.d_key,
+                     p.d_count);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
+                     nbins_tot);
+  hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
+                     p.d_cursor, p.d_sidx);
+  hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + 3) / 4), dim3(256), 0, st, nbins_tot, p.d_start,
+                     p.d_sidx);
+}
+
+template <typename T>
+static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
+  using V = typename C2<T>::t;
+  Geo &g = p.geo;
+  hipStream_t st = p.stream;
+  sort_atoms<T>(p, d_pos, false);
   {
     StageTimer t(p, 1);
     hipLaunchKernelGGL(k_spread_z<T>, dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2],
-                       p.d_X, p.d_sidx, p.d_start, p.d_species, (const V *)p.d_tw[2],
-                       (V *)p.d_W1);
+                       p.d_i0, (const T *)p.d_phi, p.d_sidx, p.d_start, p.d_species,
+                       (const V *)p.d_tw[2], (V *)p.d_W1);
   }
   {
     StageTimer t(p, 2);
@@ -1011,7 +1248,8 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
 int launch_finalize(Plan &p, double *frame_out, int accumulate) {
   const int ntot = p.geo.ncomp * p.geo.nq;
   StageTimer t(p, 4);
-  hipLaunchKernelGGL(k_finalize, dim3((ntot + 255) / 256), dim3(256), 0, p.stream, p.d_wgpart,
+  hipLaunchKernelGGL(k_finalize, dim3((ntot + kFinEntries - 1) / kFinEntries), dim3(256), 0,
+                     p.stream, p.d_wgpart,
                      p.nblk3, ntot, p.d_acc, frame_out, accumulate);
   SQC_CHECK(hipGetLastError());
   return 0;
@@ -1028,22 +1266,11 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   using V = typename C2<T>::t;
   Geo &g = p.geo;
   hipStream_t st = p.stream;
-  const int nbins_tot = g.F * g.nbins;
   const int blk = 256;
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
   const int Fa = g.F;
   // the atoms of this frame, sorted exactly as in the forward pass
-  {
-    StageTimer t(p, 0);
-    hipLaunchKernelGGL(k_prep, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X, p.d_key,
-                       p.d_count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
-                       nbins_tot);
-    hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
-                       p.d_cursor, p.d_sidx);
-    hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + blk - 1) / blk), dim3(blk), 0, st, nbins_tot,
-                       p.d_start, p.d_sidx);
-  }
+  sort_atoms<T>(p, d_pos, true);
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
   {
     StageTimer t(p, 5);
@@ -1061,13 +1288,14 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   {
     StageTimer t(p, 7);
     hipLaunchKernelGGL(k_adj_z_gather<T>, dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1, st, g,
-                       p.ax[2], (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_X, p.d_sidx,
-                       p.d_start, p.d_slots);
+                       p.ax[2], (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_i0,
+                       (const T *)p.d_phi, (const T *)p.d_dphi, p.d_sidx, p.d_start,
+                       p.d_slots);
   }
   {
     StageTimer t(p, 8);
-    hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_X, p.d_sidx, p.d_start,
-                       p.d_species, p.d_slots, Fa, grad);
+    hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_i0, p.d_species, p.d_slots,
+                       Fa, grad);
   }
   SQC_CHECK(hipGetLastError());
   return 0;
@@ -1079,24 +1307,30 @@ int launch_adjoint(Plan &p, const double *d_pos, const void *rho, const double *
                      : adjoint_t<double>(p, d_pos, rho, gS, gP, grad);
 }
 
-// Raise the dynamic-LDS ceiling of the kernels that use more than 64 KiB.
-void kernel_lds_limits(Plan &p) {
-  const int mx = 160 * 1024;
-  if (p.prec == 1) {
-    (void)hipFuncSetAttribute((const void *)k_spread_z<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_fft_y<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_fft_x_bin<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_adj_x<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_adj_y<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_adj_z_gather<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-  } else {
-    (void)hipFuncSetAttribute((const void *)k_spread_z<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_fft_y<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_fft_x_bin<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_adj_x<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_adj_y<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute((const void *)k_adj_z_gather<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+// Raise the dynamic-LDS ceiling of each kernel to what the plan needs.
+template <typename T>
+static int lds_limits_t(Plan &p) {
+  struct K {
+    const void *f;
+    size_t bytes;
+  } ks[] = {
+      {(const void *)k_spread_z<T>, p.lds1}, {(const void *)k_adj_z_gather<T>, p.lds1},
+      {(const void *)k_fft_y<T>, p.lds2},    {(const void *)k_adj_y<T>, p.lds2},
+      {(const void *)k_fft_x_bin<T>, p.lds3}, {(const void *)k_adj_x<T>, p.lds3},
+  };
+  for (auto &k : ks) {
+    const hipError_t e =
+        hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return (int)e;
+    }
   }
+  return 0;
+}
+
+int kernel_lds_limits(Plan &p) {
+  return p.prec == 1 ? lds_limits_t<float>(p) : lds_limits_t<double>(p);
 }
 
 }  // namespace sqc
