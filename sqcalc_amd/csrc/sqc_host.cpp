@@ -494,6 +494,13 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
     g.ty = cand[pick][1];
     p.lds1 = ((size_t)(g.txr / 2) * g.ty * g.cs + n3) * vs + sqc::kStageCap * (16 + 2 * 4);
   }
+  auto ilog2 = [](int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+  };
+  g.ltxr = ilog2(g.txr);
+  g.lty = ilog2(g.ty);
   g.ntx = (n1 + g.txr - 1) / g.txr;
   g.nty = (n2 + g.ty - 1) / g.ty;
   g.nbins = g.ntx * g.nty;
@@ -506,6 +513,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
     int tx2 = 16;
     while (tx2 > 1 && ((size_t)tx2 * g.ls2 + n2) * vs > lds_pref) tx2 /= 2;
     p.tx2 = tx2;
+    g.ltx2 = ilog2(tx2);
     p.lds2 = ((size_t)tx2 * g.ls2 + n2) * vs;
   }
   {

@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "sqc_internal.hpp"
 
@@ -182,8 +183,8 @@ __device__ __forceinline__ void wave_stage(typename C2<T>::t *line, int n, int N
 }
 
 template <typename T>
-__device__ void wave_fft(typename C2<T>::t *line, const AxisPlan &ax, const typename C2<T>::t *tw,
-                         int lane) {
+__device__ void wave_fft_any(typename C2<T>::t *line, const AxisPlan &ax,
+                             const typename C2<T>::t *tw, int lane) {
   int Ns = 1;
   for (int s = 0; s < ax.nst; ++s) {
     const int R = ax.rad[s];
@@ -196,6 +197,83 @@ __device__ void wave_fft(typename C2<T>::t *line, const AxisPlan &ax, const type
     }
     Ns *= R;
   }
+}
+
+// Compile-time line DFT for the fine-grid sizes the benchmark workloads use:
+// the radix plan, strides and divisors are constants, so the LDS accesses of
+// a butterfly become one address plus immediate offsets.
+struct RadixPlan {
+  int nst;
+  int rad[kMaxStages];
+};
+
+constexpr RadixPlan radix_plan(int n) {
+  RadixPlan p{0, {}};
+  const int order[5] = {8, 5, 4, 3, 2};
+  for (int i = 0; i < 5; ++i)
+    while (n % order[i] == 0) {
+      p.rad[p.nst++] = order[i];
+      n /= order[i];
+    }
+  return p;
+}
+
+template <typename T, int N, int R, int Ns>
+__device__ __forceinline__ void fixed_stage(typename C2<T>::t *line, const typename C2<T>::t *tw,
+                                            int lane) {
+  using V = typename C2<T>::t;
+  constexpr int nb = N / R;
+  constexpr int MAXB = (nb + 63) / 64;
+  constexpr int tstep = N / (Ns * R);
+  V v[MAXB][R];
+#pragma unroll
+  for (int it = 0; it < MAXB; ++it) {
+    const int j = lane + 64 * it;
+    if (it + 1 < MAXB || nb % 64 == 0 || j < nb) {
+      const V *src = line + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = src[r * nb];
+      if constexpr (Ns > 1) {
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[it][r] = cmul(v[it][r], tw[k * (r * tstep)]);
+      }
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int it = 0; it < MAXB; ++it) {
+    const int j = lane + 64 * it;
+    if (it + 1 < MAXB || nb % 64 == 0 || j < nb) {
+      Dft<T, R>::run(v[it]);
+      V *dst = line + (j / Ns) * (Ns * R) + j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * Ns] = v[it][r];
+    }
+  }
+  wave_sync();
+}
+
+template <typename T, int N, int S, int Ns>
+__device__ __forceinline__ void fixed_stages(typename C2<T>::t *line, const typename C2<T>::t *tw,
+                                             int lane) {
+  constexpr RadixPlan P = radix_plan(N);
+  if constexpr (S < P.nst) {
+    constexpr int R = P.rad[S];
+    fixed_stage<T, N, R, Ns>(line, tw, lane);
+    fixed_stages<T, N, S + 1, Ns * R>(line, tw, lane);
+  }
+}
+
+// Line DFT of the plan's length: compile-time when N > 0 (N == ax.n),
+// otherwise the runtime-planned stages.
+template <typename T, int N>
+__device__ __forceinline__ void wave_fft(typename C2<T>::t *line, const AxisPlan &ax,
+                                         const typename C2<T>::t *tw, int lane) {
+  if constexpr (N > 0)
+    fixed_stages<T, N, 0, 1>(line, tw, lane);
+  else
+    wave_fft_any<T>(line, ax, tw, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -516,7 +594,7 @@ __device__ void slab_lists(Stage &st, const int4 *rec, unsigned short *slist, in
 // and walks the staged atoms in order, so every LDS cell is updated by one
 // wave, in atom order: no atomics, and the grid is bit-reproducible.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 *__restrict__ i0a,
                                                  const T *__restrict__ phi,
                                                  const int32_t *__restrict__ sidx,
@@ -606,16 +684,18 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     __syncthreads();
   }
   __syncthreads();
-  for (int col = wave; col < cc; col += NW) wave_fft<T>(tile + (size_t)col * cs, az, tw, lane);
+  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, tw, lane);
   __syncthreads();
   // split the packed pairs and write h3 = 0..M3
   const int M3 = g.M[2];
-  const int per = tyw * txw;
+  // txr and ty are powers of two: the tile index splits with shifts
+  const int per = g.ty << g.ltxr;
   for (int idx = tid; idx < (M3 + 1) * per; idx += NT) {
-    const int xl = idx % txw;
-    const int r = idx / txw;
-    const int yl = r % tyw;
-    const int k3 = r / tyw;
+    const int xl = idx & (g.txr - 1);
+    const int r = idx >> g.ltxr;
+    const int yl = r & (g.ty - 1);
+    const int k3 = r >> g.lty;
+    if (xl >= txw || yl >= tyw) continue;
     const int col = yl * ccx + (xl >> 1);
     const V Z = tile[(size_t)col * cs + (k3)];
     const V Zm = tile[(size_t)col * cs + ((n3 - k3) % n3)];
@@ -632,7 +712,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
 // y-DFT.  One workgroup per (x-chunk, h3, field); lines are transposed into
 // contiguous LDS rows (stride n2 + 1) and transformed one per wave.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
                                               const typename C2<T>::t *__restrict__ W1,
                                               const typename C2<T>::t *__restrict__ twy,
@@ -651,25 +731,26 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
   if (hi < lo) return;
   for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
   const V *src = W1 + ((size_t)f * (M3 + 1) + k3) * n2 * n1;
+  const int lt = g.ltx2, tm = tx2 - 1;
   batched_copy<8, V>(
       n2 * tx2,
       [&](int idx) {
-        const int x = x0 + idx % tx2, y = idx / tx2;
+        const int x = x0 + (idx & tm), y = idx >> lt;
         return x < n1 ? src[(size_t)y * n1 + x] : V{T(0), T(0)};
       },
-      [&](int idx, V v) { lines[(idx % tx2) * ls + (idx / tx2)] = v; });
+      [&](int idx, V v) { lines[(idx & tm) * ls + (idx >> lt)] = v; });
   __syncthreads();
-  for (int l = wave; l < tx2; l += NW) wave_fft<T>(lines + (size_t)l * ls, ay, tw, lane);
+  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, tw, lane);
   __syncthreads();
   const int nl = hi - lo + 1;
   V *dst = W2 + ((size_t)f * g.nlines + loff[k3]) * n1;
   for (int idx = tid; idx < nl * tx2; idx += NT) {
-    const int xl = idx % tx2, r = idx / tx2;
+    const int xl = idx & tm, r = idx >> lt;
     const int x = x0 + xl;
     if (x >= n1) continue;
     const int h2 = lo + r;
     const int yi = h2 < 0 ? h2 + n2 : h2;
-    dst[(size_t)r * n1 + x] = lines[xl * ls + (yi)];
+    dst[(size_t)r * n1 + x] = lines[xl * ls + yi];
   }
 }
 
@@ -718,7 +799,7 @@ __device__ __forceinline__ void load_group(LineGroup &lg, int l0, int nl,
   }
 }
 
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, int ngroups,
                                                   const typename C2<T>::t *__restrict__ W2,
                                                   const typename C2<T>::t *__restrict__ twx,
@@ -750,15 +831,20 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
     const int nl = min(lb, g.nlines - l0);
     __syncthreads();
     load_group(lg, l0, nl, line_info, line_off, ph2, ph3);
+    const float inv_n1 = 1.0f / (float)n1, inv_f = 1.0f / (float)F;
     batched_copy<8, V>(
         nl * F * n1,
         [&](int idx) {
-          const int x = idx % n1, r = idx / n1;
-          return W2[((size_t)(r % F) * g.nlines + l0 + r / F) * n1 + x];
+          const int r = small_div(idx, inv_n1), x = idx - r * n1;
+          const int li = small_div(r, inv_f), f = r - li * F;
+          return W2[((size_t)f * g.nlines + l0 + li) * n1 + x];
         },
-        [&](int idx, V v) { rows[(idx / n1) * rs + (idx % n1)] = v; });
+        [&](int idx, V v) {
+          const int r = small_div(idx, inv_n1);
+          rows[r * rs + (idx - r * n1)] = v;
+        });
     __syncthreads();
-    for (int r = wave; r < nl * F; r += NW) wave_fft<T>(rows + (size_t)r * rs, ax1, tw, lane);
+    for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
     __syncthreads();
     const int c0 = lg.off[0], c1 = lg.off[nl];
     for (int c = c0 + tid; c < c1; c += NT) {
@@ -862,7 +948,7 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ wgp
 // Adjoint.  V_c(x) = sum_m C_cm exp(i h_m . x) is real; its gradient at the
 // atoms is interpolated with the kernel derivative.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int ngroups, int Fa,
                                               const typename C2<T>::t *__restrict__ rho,
                                               int64_t ncand,
@@ -941,7 +1027,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       }
     }
     __syncthreads();
-    for (int r = wave; r < nl * Fa; r += NW) wave_fft<T>(rows + (size_t)r * rs, ax1, tw, lane);
+    for (int r = wave; r < nl * Fa; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
     __syncthreads();
     for (int idx = tid; idx < nl * Fa * n1; idx += NT) {
       const int x = idx % n1, r = idx / n1;
@@ -951,7 +1037,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
   }
 }
 
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
                                               const typename C2<T>::t *__restrict__ W2,
                                               const typename C2<T>::t *__restrict__ twy,
@@ -971,26 +1057,27 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
   for (int i = tid; i < tx2 * ls; i += NT) lines[i] = V{T(0), T(0)};
   __syncthreads();
   const V *src = W2 + ((size_t)cf * g.nlines + (hi >= lo ? loff[k3] : 0)) * n1;
+  const int lt = g.ltx2, tm = tx2 - 1;
   batched_copy<8, V>(
       (hi - lo + 1) * tx2,
       [&](int idx) {
-        const int x = x0 + idx % tx2;
-        return x < n1 ? src[(size_t)(idx / tx2) * n1 + x] : V{T(0), T(0)};
+        const int x = x0 + (idx & tm);
+        return x < n1 ? src[(size_t)(idx >> lt) * n1 + x] : V{T(0), T(0)};
       },
       [&](int idx, V v) {
-        const int h2 = lo + idx / tx2;
+        const int h2 = lo + (idx >> lt);
         const int yi = h2 < 0 ? h2 + n2 : h2;
-        if (x0 + idx % tx2 < n1) lines[(idx % tx2) * ls + (yi)] = v;
+        if (x0 + (idx & tm) < n1) lines[(idx & tm) * ls + yi] = v;
       });
   __syncthreads();
-  for (int l = wave; l < tx2; l += NW) wave_fft<T>(lines + (size_t)l * ls, ay, tw, lane);
+  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, tw, lane);
   __syncthreads();
   V *dst = W1 + ((size_t)cf * (M3 + 1) + k3) * n2 * n1;
   for (int idx = tid; idx < n2 * tx2; idx += NT) {
-    const int xl = idx % tx2, y = idx / tx2;
+    const int xl = idx & tm, y = idx >> lt;
     const int x = x0 + xl;
     if (x >= n1) continue;
-    V v = lines[xl * ls + (y)];
+    V v = lines[xl * ls + y];
     // the h3 = 0 plane holds only the canonical half of (h1, h2): the full
     // plane sum is that half plus its conjugate
     if (k3 == 0) v = V{T(2) * v.x, T(0)};
@@ -998,7 +1085,7 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
   }
 }
 
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
                                                      const typename C2<T>::t *__restrict__ W1,
                                                      const typename C2<T>::t *__restrict__ twz,
@@ -1045,7 +1132,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         if (k3 > 0) tile[(size_t)col * cs + (n3 - k3)] = V{A.x + B.y, B.x - A.y};
       });
   __syncthreads();
-  for (int col = wave; col < cc; col += NW) wave_fft<T>(tile + (size_t)col * cs, az, tw, lane);
+  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, tw, lane);
   stage_window(st, g, tx, ty, cf, start);
 
   const int ns = g.ns;
@@ -1195,6 +1282,31 @@ int collect_stage_times(Plan &p) {
     if (e_ != hipSuccess) return (int)e_;    \
   } while (0)
 
+
+// Sizes with a compile-time line DFT: the fine grids of the benchmark
+// workloads (sigma 1.25 in FP32, sigma 2 in FP64); every other size runs the
+// runtime-planned DFT (N = 0).
+#define SQC_FIXED_F32(X) X(90) X(120) X(270) X(432) X(480)
+#define SQC_FIXED_F64(X) X(144) X(180) X(432) X(648) X(750)
+
+template <typename T> struct FixedSizes;
+template <> struct FixedSizes<float> {
+  template <typename F> static void pick(int n, F &&f) {
+#define SQC_CASE(N) if (n == N) return f(std::integral_constant<int, N>{});
+    SQC_FIXED_F32(SQC_CASE)
+#undef SQC_CASE
+    f(std::integral_constant<int, 0>{});
+  }
+};
+template <> struct FixedSizes<double> {
+  template <typename F> static void pick(int n, F &&f) {
+#define SQC_CASE(N) if (n == N) return f(std::integral_constant<int, N>{});
+    SQC_FIXED_F64(SQC_CASE)
+#undef SQC_CASE
+    f(std::integral_constant<int, 0>{});
+  }
+};
+
 template <typename T>
 static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
   Geo &g = p.geo;
@@ -1222,24 +1334,31 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
   sort_atoms<T>(p, d_pos, false);
   {
     StageTimer t(p, 1);
-    hipLaunchKernelGGL(k_spread_z<T>, dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2],
-                       p.d_i0, (const T *)p.d_phi, p.d_sidx, p.d_start, p.d_species,
-                       (const V *)p.d_tw[2], (V *)p.d_W1);
+    FixedSizes<T>::pick(g.n[2], [&](auto n) {
+      hipLaunchKernelGGL((k_spread_z<T, decltype(n)::value>), dim3(g.ntx * g.nty, g.F), dim3(NT),
+                         p.lds1, st, g, p.ax[2], p.d_i0, (const T *)p.d_phi, p.d_sidx, p.d_start,
+                         p.d_species, (const V *)p.d_tw[2], (V *)p.d_W1);
+    });
   }
   {
     StageTimer t(p, 2);
-    hipLaunchKernelGGL(k_fft_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, g.F),
-                       dim3(NT), p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W1,
-                       (const V *)p.d_tw[1], p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W2);
+    FixedSizes<T>::pick(g.n[1], [&](auto n) {
+      hipLaunchKernelGGL((k_fft_y<T, decltype(n)::value>),
+                         dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, g.F), dim3(NT), p.lds2, st,
+                         g, p.ax[1], p.tx2, (const V *)p.d_W1, (const V *)p.d_tw[1], p.d_lo2,
+                         p.d_hi2, p.d_loff, (V *)p.d_W2);
+    });
   }
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
   {
     StageTimer t(p, 3);
-    hipLaunchKernelGGL(k_fft_x_bin<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb,
-                       ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
-                       (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0],
-                       p.d_ph[1], p.d_ph[2], p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
-                       mode == 0 ? p.d_gnum : nullptr);
+    FixedSizes<T>::pick(g.n[0], [&](auto n) {
+      hipLaunchKernelGGL((k_fft_x_bin<T, decltype(n)::value>), dim3(p.nblk3), dim3(NT), p.lds3,
+                         st, g, p.ax[0], p.lb, ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
+                         (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0],
+                         p.d_ph[1], p.d_ph[2], p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
+                         mode == 0 ? p.d_gnum : nullptr);
+    });
   }
   SQC_CHECK(hipGetLastError());
   return launch_finalize(p, frame_out, mode == 0);
@@ -1274,23 +1393,31 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
   {
     StageTimer t(p, 5);
-    hipLaunchKernelGGL(k_adj_x<T>, dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
-                       Fa, (const V *)rho, (int64_t)p.ncand, (const V *)p.d_tw[0],
-                       (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0], p.d_ph[1],
-                       p.d_ph[2], p.d_den, p.d_cnt, gS, gP, (V *)p.d_W2);
+    FixedSizes<T>::pick(g.n[0], [&](auto n) {
+      hipLaunchKernelGGL((k_adj_x<T, decltype(n)::value>), dim3(p.nblk3), dim3(NT), p.lds3, st, g,
+                         p.ax[0], p.lb, ngroups, Fa, (const V *)rho, (int64_t)p.ncand,
+                         (const V *)p.d_tw[0], (const int4 *)p.d_line_info, p.d_line_off,
+                         p.d_stab, p.d_ph[0], p.d_ph[1], p.d_ph[2], p.d_den, p.d_cnt, gS, gP,
+                         (V *)p.d_W2);
+    });
   }
   {
     StageTimer t(p, 6);
-    hipLaunchKernelGGL(k_adj_y<T>, dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, Fa), dim3(NT),
-                       p.lds2, st, g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1],
-                       p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W1);
+    FixedSizes<T>::pick(g.n[1], [&](auto n) {
+      hipLaunchKernelGGL((k_adj_y<T, decltype(n)::value>),
+                         dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, Fa), dim3(NT), p.lds2, st,
+                         g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1], p.d_lo2,
+                         p.d_hi2, p.d_loff, (V *)p.d_W1);
+    });
   }
   {
     StageTimer t(p, 7);
-    hipLaunchKernelGGL(k_adj_z_gather<T>, dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1, st, g,
-                       p.ax[2], (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_i0,
-                       (const T *)p.d_phi, (const T *)p.d_dphi, p.d_sidx, p.d_start,
-                       p.d_slots);
+    FixedSizes<T>::pick(g.n[2], [&](auto n) {
+      hipLaunchKernelGGL((k_adj_z_gather<T, decltype(n)::value>), dim3(g.ntx * g.nty, Fa),
+                         dim3(NT), p.lds1, st, g, p.ax[2], (const V *)p.d_W1,
+                         (const V *)p.d_tw[2], p.d_i0, (const T *)p.d_phi, (const T *)p.d_dphi,
+                         p.d_sidx, p.d_start, p.d_slots);
+    });
   }
   {
     StageTimer t(p, 8);
@@ -1310,23 +1437,29 @@ int launch_adjoint(Plan &p, const double *d_pos, const void *rho, const double *
 // Raise the dynamic-LDS ceiling of each kernel to what the plan needs.
 template <typename T>
 static int lds_limits_t(Plan &p) {
-  struct K {
-    const void *f;
-    size_t bytes;
-  } ks[] = {
-      {(const void *)k_spread_z<T>, p.lds1}, {(const void *)k_adj_z_gather<T>, p.lds1},
-      {(const void *)k_fft_y<T>, p.lds2},    {(const void *)k_adj_y<T>, p.lds2},
-      {(const void *)k_fft_x_bin<T>, p.lds3}, {(const void *)k_adj_x<T>, p.lds3},
-  };
-  for (auto &k : ks) {
+  int err = 0;
+  auto set = [&](const void *f, size_t bytes) {
+    if (err) return;
     const hipError_t e =
-        hipFuncSetAttribute(k.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.bytes);
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
-      return (int)e;
+      err = (int)e;
     }
-  }
-  return 0;
+  };
+  FixedSizes<T>::pick(p.geo.n[2], [&](auto n) {
+    set((const void *)k_spread_z<T, decltype(n)::value>, p.lds1);
+    set((const void *)k_adj_z_gather<T, decltype(n)::value>, p.lds1);
+  });
+  FixedSizes<T>::pick(p.geo.n[1], [&](auto n) {
+    set((const void *)k_fft_y<T, decltype(n)::value>, p.lds2);
+    set((const void *)k_adj_y<T, decltype(n)::value>, p.lds2);
+  });
+  FixedSizes<T>::pick(p.geo.n[0], [&](auto n) {
+    set((const void *)k_fft_x_bin<T, decltype(n)::value>, p.lds3);
+    set((const void *)k_adj_x<T, decltype(n)::value>, p.lds3);
+  });
+  return err;
 }
 
 int kernel_lds_limits(Plan &p) {
