@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
 
 
 def main():
@@ -41,6 +42,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip forward + adjoint")
     ap.add_argument("--fp64-frames", type=int, default=10)
     ap.add_argument("--adj-frames", type=int, default=20)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run an elementwise copy of 1 GiB (reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -55,6 +59,13 @@ def main():
     from sqcalc_amd.synthetic import config
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     wl = config("c2")
     fps = args.frames_per_step
     nbatch = max(1, wl.nframes // fps)
@@ -78,6 +89,10 @@ def main():
         sf.accumulate_device(pos[(w % nbatch) * fps:(w % nbatch + 1) * fps])
     torch.cuda.synchronize()
     sf.fetch()
+    # HIP events around every stage launch on the plan's own stream, over the
+    # timed steps themselves
+    sf.profile(True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         b = k % nbatch
@@ -87,10 +102,6 @@ def main():
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     frames_per_s = fps * args.steps / (t1 - t0)
-
-    # per-stage event timing on the plan stream, one extra step
-    sf.profile(True)
-    sf.accumulate_device(pos[0:fps])
     stages = sf.stage_times()
     sf.profile(False)
     nbytes = sf.stage_bytes()
@@ -105,6 +116,7 @@ def main():
     stage_ms = {k: round(v[0] / v[1], 4) for k, v in stages.items()}
     # the whole frame against the same byte model
     frame_bytes = sum(nbytes[k] for k in ("spread_z", "dft_y", "dft_x_bin"))
+    ref_bytes = _full_grid_bytes(info, wl.natoms, 8)
     sf.close()
 
     # ---- plain FP64 forward (sigma 2) ----
@@ -172,6 +184,8 @@ def main():
         "roofline": roofline,
         "frame_hbm_model_gb": round(frame_bytes / 1e9, 3),
         "frame_frac_of_hbm_peak": round(frame_bytes * frames_per_s / 1e9 / HBM_PEAK_GBS, 4),
+        "full_grid_model_gb": round(ref_bytes / 1e9, 3),
+        "full_grid_equiv_frac": round(ref_bytes * frames_per_s / 1e9 / HBM_PEAK_GBS, 4),
         "stage_ms": stage_ms,
         "fp64_plain_frames_per_s": round(fp64, 2) if fp64 else None,
         "fp32_over_fp64": round(frames_per_s / fp64, 2) if fp64 else None,
@@ -180,6 +194,17 @@ def main():
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _full_grid_bytes(info, natoms, b):
+    """Bytes per frame of the unfused pipeline (full fine grid written by the
+    spreader, pruned row-column FFT with one HBM round trip per pass, kept
+    modes stored): S b (2 Gf + 2 m1 n2 n3 + 2 m1 m2 n3 + K) + N (3*8 + 4)."""
+    m1, m2, _ = info.modes
+    n1, n2, n3 = info.fine
+    gf = n1 * n2 * n3
+    return (info.nfields * b * (2 * gf + 2 * m1 * n2 * n3 + 2 * m1 * m2 * n3 + info.nmodes)
+            + natoms * 28)
 
 
 def _committed_traffic(kernel):

@@ -266,6 +266,12 @@ void free_plan(sqc_hip_plan *h) {
                   h->g_grad, h->g_rho};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  if (!p.pending.empty()) (void)hipStreamSynchronize(p.stream);
+  for (auto &q : p.pending) {
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  for (hipEvent_t e : p.ev_free) (void)hipEventDestroy(e);
   if (p.ev_in) (void)hipEventDestroy(p.ev_in);
   if (p.ev_out) (void)hipEventDestroy(p.ev_out);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);

@@ -132,6 +132,7 @@ struct Plan {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_free;  // recycled timing events
   double stage_ms[kNumStages] = {};
   int64_t stage_n[kNumStages] = {};
 };

@@ -522,8 +522,13 @@ __device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0,
   int flag = 0;
   int4 r = int4{0, 0, 0, 0};
   if (t < st.total) {
-    int b = 0;
-    while (st.bin_pref[b + 1] <= t) ++b;
+    // the last bin whose first candidate is at or before t
+    int b = 0, hi = st.nbin - 1;
+    while (b < hi) {
+      const int mid = (b + hi + 1) >> 1;
+      if (st.bin_pref[mid] <= t) b = mid;
+      else hi = mid - 1;
+    }
     const int j = sidx[st.bin_first[b] + (t - st.bin_pref[b])];
     const int4 i0 = i0a[j];
     int kx = 0, xl = 0, ky = 0, yl = 0;
@@ -586,6 +591,35 @@ __device__ void slab_lists(Stage &st, const int4 *rec, unsigned short *slist, in
   __syncthreads();
 }
 
+// Add the z points [z0, z0 + pc) of one atom's stencil into the tile: cell
+// c = xy * pc + kz, two cells per lane per round with both reads issued
+// before the writes (the cells of one atom are distinct).  fz of point kz is
+// lane fzl + kz of ph; fxy and the column offset of xy cell xy are lane xy
+// of fxy and cofs.
+template <typename T>
+__device__ __forceinline__ void spread_piece(T *tflat, T ph, T fxy, int cofs, int oxy, int fzl,
+                                             int z0, int pc, int lane) {
+  if (pc <= 0) return;
+  const int cells = oxy * pc;
+  const float inv_pc = __builtin_amdgcn_rcpf((float)pc);
+  for (int c0 = 0; c0 < cells; c0 += 128) {
+    const int ca = c0 + lane, cb = ca + 64;
+    const int va = ca < cells ? ca : 0, vb = cb < cells ? cb : 0;
+    const int xa = small_div(va, inv_pc), xb = small_div(vb, inv_pc);
+    const int za = va - xa * pc, zb = vb - xb * pc;
+    const T fa = __shfl(fxy, xa) * __shfl(ph, fzl + za);
+    const T fb = __shfl(fxy, xb) * __shfl(ph, fzl + zb);
+    const int oa = __shfl(cofs, xa) + 2 * (z0 + za);
+    const int ob = __shfl(cofs, xb) + 2 * (z0 + zb);
+    const bool wa = ca < cells, wb = cb < cells;
+    T ra = T(0), rb = T(0);
+    if (wa) ra = tflat[oa];
+    if (wb) rb = tflat[ob];
+    if (wa) tflat[oa] = ra + fa;
+    if (wb) tflat[ob] = rb + fb;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Spread + z-DFT.  One workgroup per (x,y) tile of txr x ty real columns and
 // one field.  The tile is held in LDS as txr/2 x ty complex columns of n3
@@ -642,40 +676,29 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
       const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
       const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
       const int a = r.w;
-      int pk[2], pz[2], pc[2], np = 0;
-      {
-        const int e1 = min(a + ns, n3);
-        const int lo = max(a, zs0), hi = min(e1, zs1);
-        if (lo < hi) { pk[np] = lo - a; pz[np] = lo; pc[np] = hi - lo; ++np; }
-        if (a + ns > n3) {
-          const int e2 = a + ns - n3;
-          const int lo2 = max(0, zs0), hi2 = min(e2, zs1);
-          if (lo2 < hi2) { pk[np] = n3 - a + lo2; pz[np] = lo2; pc[np] = hi2 - lo2; ++np; }
-        }
-      }
+      // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
+      // (ox <= 16 and oy <= 2 by the tile choice, so ox*oy <= 64)
       const T w = (g.F == 1) ? (T)g.amp[species[j]] : T(1);
-      const float inv_ox = 1.0f / (float)ox;
-      for (int p = 0; p < np; ++p) {
-        const int cells = ox * oy * pc[p];
-        const float inv_pc = 1.0f / (float)pc[p];
-        for (int c0 = 0; c0 < cells; c0 += 64) {
-          // every lane takes part in the shuffles; only c < cells writes
-          const int c = c0 + lane;
-          const int cv = c < cells ? c : 0;
-          const int t1 = small_div(cv, inv_pc);
-          const int kz = cv - t1 * pc[p];
-          const int iy = small_div(t1, inv_ox);
-          const int ix = t1 - iy * ox;
-          const T fz = __shfl(ph, 2 * ns + pk[p] + kz);
-          const T fx = __shfl(ph, kxl + ix);
-          const T fy = __shfl(ph, ns + kyl + iy);
-          if (c < cells) {
-            const int xl = xll + ix, yl = yll + iy, z = pz[p] + kz;
-            const int col = yl * ccx + (xl >> 1);
-            T *cell = tflat + 2 * ((size_t)col * cs + (z)) + (xl & 1);
-            *cell += w * fx * fy * fz;
-          }
-        }
+      T fxy;
+      int cofs;
+      {
+        const int oxy = ox * oy;
+        const int l = lane < oxy ? lane : 0;
+        const int iy = small_div(l, __builtin_amdgcn_rcpf((float)ox));
+        const int ix = l - iy * ox;
+        fxy = w * __shfl(ph, kxl + ix) * __shfl(ph, ns + kyl + iy);
+        const int xl = xll + ix, yl = yll + iy;
+        cofs = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
+      }
+      // the stencil's z range inside this wave's slab: one piece, or two
+      // when it wraps through z = 0
+      const int e1 = min(a + ns, n3);
+      spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + max(a, zs0) - a, max(a, zs0),
+                      min(e1, zs1) - max(a, zs0), lane);
+      if (a + ns > n3) {
+        const int lo2 = zs0, hi2 = min(a + ns - n3, zs1);
+        spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + n3 - a + lo2, lo2, hi2 - lo2,
+                        lane);
       }
     }
     __syncthreads();
@@ -1247,11 +1270,25 @@ struct StageTimer {
   hipEvent_t a = nullptr, b = nullptr;
   StageTimer(Plan &pl, int st) : p(pl), stage(st) {
     if (!p.profiling) return;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    a = take();
+    b = take();
+    if (!a || !b) {
+      if (a) p.ev_free.push_back(a);
+      if (b) p.ev_free.push_back(b);
       a = b = nullptr;
       return;
     }
     (void)hipEventRecord(a, p.stream);
+  }
+  hipEvent_t take() {
+    hipEvent_t e = nullptr;
+    if (!p.ev_free.empty()) {
+      e = p.ev_free.back();
+      p.ev_free.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
   }
   ~StageTimer() {
     if (!a) return;
@@ -1269,8 +1306,8 @@ int collect_stage_times(Plan &p) {
       p.stage_ms[q.stage] += ms;
       p.stage_n[q.stage] += 1;
     }
-    (void)hipEventDestroy(q.a);
-    (void)hipEventDestroy(q.b);
+    p.ev_free.push_back(q.a);
+    p.ev_free.push_back(q.b);
   }
   p.pending.clear();
   return (int)e;

@@ -1,0 +1,138 @@
+"""Short text summaries of rocprofv3 CSV output for the S(q) kernels.
+
+  python -m sqcalc_amd.rocprof_summary stats STATS_CSV --command "..." > out.md
+      per-kernel calls / total / average duration from a
+      ``rocprofv3 --kernel-trace --stats --output-format csv`` run
+      (the ``*_kernel_stats.csv`` file).
+
+  python -m sqcalc_amd.rocprof_summary traffic --fetch FETCH_CSV --write WRITE_CSV
+         --copy-bytes B --json out.json --command "..." > out.md
+      HBM bytes per dispatch from two ``rocprofv3 --pmc FETCH_SIZE`` and
+      ``--pmc WRITE_SIZE`` runs (``*_counter_collection.csv``, KiB per
+      dispatch).  Each counter is scaled by the factor that makes it read
+      B bytes for the calibration copy kernel (torch's elementwise ``neg``
+      over B bytes in and B bytes out, launched by ``bench.py
+      --calibrate-copy``), and the JSON maps each stage name to the scaled
+      fetch + write bytes per launch.
+"""
+
+import argparse
+import collections
+import csv
+import json
+import re
+import sys
+
+# kernel template name -> stage name used by the plan's stage timers
+STAGE_OF = {
+    "k_prep": "atom_sort", "k_scan": "atom_sort", "k_scatter": "atom_sort",
+    "k_binsort": "atom_sort", "k_spread_z": "spread_z", "k_fft_y": "dft_y",
+    "k_fft_x_bin": "dft_x_bin", "k_finalize": "finalize", "k_adj_x": "adj_x",
+    "k_adj_y": "adj_y", "k_adj_z_gather": "adj_z_gather", "k_adj_sum": "adj_sum",
+}
+COPY_PATTERN = "neg"
+
+
+def short_name(name):
+    """'void sqc::k_spread_z<float, 480>(sqc::Geo, ...)' -> 'k_spread_z<float, 480>'."""
+    n = name.split("(")[0].replace("void ", "").replace("sqc::", "").strip()
+    return n
+
+
+def base_name(name):
+    return re.split(r"[<(]", short_name(name))[0]
+
+
+def is_copy(name):
+    return COPY_PATTERN in name and "elementwise" in name
+
+
+def read_stats(path):
+    rows = []
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            rows.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]),
+                         float(r["AverageNs"]), float(r["Percentage"])))
+    return rows
+
+
+def read_counter(path, counter):
+    """Mean counter value per dispatch, per kernel name."""
+    acc = collections.defaultdict(list)
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            if r["Counter_Name"] != counter:
+                continue
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def cmd_stats(args):
+    rows = read_stats(args.csv)
+    print("# rocprofv3 kernel statistics\n")
+    print("Command:\n\n    %s\n" % args.command)
+    print("| kernel | calls | total ms | average us | % |")
+    print("|---|---:|---:|---:|---:|")
+    for name, calls, tot, avg, pct in rows[:args.top]:
+        print("| `%s` | %d | %.3f | %.2f | %.2f |" % (short_name(name), calls, tot / 1e6,
+                                                     avg / 1e3, pct))
+
+
+def cmd_traffic(args):
+    fetch = read_counter(args.fetch, "FETCH_SIZE")
+    write = read_counter(args.write, "WRITE_SIZE")
+    copies_f = [v for k, v in fetch.items() if is_copy(k)]
+    copies_w = [v for k, v in write.items() if is_copy(k)]
+    if not copies_f or not copies_w:
+        sys.exit("no calibration copy kernel in the counter files")
+    kib = args.copy_bytes / 1024.0
+    sf = kib / copies_f[0]
+    sw = kib / copies_w[0]
+    out = {"_calibration": {"copy_bytes": args.copy_bytes, "fetch_kib": copies_f[0],
+                            "write_kib": copies_w[0], "fetch_scale": sf, "write_scale": sw}}
+    print("# HBM traffic per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE)\n")
+    print("Commands:\n\n    %s\n" % args.command.replace(" && ", "\n    "))
+    print("Calibration: elementwise copy of %d bytes in and out read FETCH_SIZE %.0f KiB, "
+          "WRITE_SIZE %.0f KiB; scale factors fetch %.4f, write %.4f.\n"
+          % (args.copy_bytes, copies_f[0], copies_w[0], sf, sw))
+    print("| kernel | stage | fetch MB | write MB | total MB (scaled) |")
+    print("|---|---|---:|---:|---:|")
+    per_stage = collections.defaultdict(float)
+    for k in sorted(fetch, key=lambda k: -fetch[k]):
+        b = base_name(k)
+        if b not in STAGE_OF or k not in write:
+            continue
+        fb = fetch[k] * 1024.0 * sf
+        wb = write[k] * 1024.0 * sw
+        st = STAGE_OF[b]
+        per_stage[st] += fb + wb
+        print("| `%s` | %s | %.1f | %.1f | %.1f |" % (short_name(k), st, fb / 1e6, wb / 1e6,
+                                                     (fb + wb) / 1e6))
+    for st, v in per_stage.items():
+        out[st] = int(round(v))
+    if args.json:
+        with open(args.json, "w") as fh:
+            json.dump(out, fh, indent=1)
+            fh.write("\n")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("--command", default="")
+    a.add_argument("--top", type=int, default=20)
+    b = sub.add_parser("traffic")
+    b.add_argument("--fetch", required=True)
+    b.add_argument("--write", required=True)
+    b.add_argument("--copy-bytes", type=int, required=True)
+    b.add_argument("--json")
+    b.add_argument("--command", default="")
+    args = ap.parse_args(argv)
+    cmd_stats(args) if args.cmd == "stats" else cmd_traffic(args)
+
+
+if __name__ == "__main__":
+    main()
