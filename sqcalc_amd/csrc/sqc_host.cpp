@@ -256,6 +256,92 @@ double amplitude(const sqc::Geo &g, int sp, double q) {
   return g.amp[sp];
 }
 
+// Spread tiles (txr real x-columns, ty y-columns), largest first.  txr <= 16
+// and ty <= 2 keep a stencil's xy footprint inside a tile at 64 cells or
+// fewer, which the spreader's per-lane xy table relies on.
+constexpr int kTiles[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {4, 2}, {4, 1}, {2, 1}};
+constexpr int kNumTiles = 7;
+constexpr size_t kLdsCap = 160 * 1024;
+constexpr size_t kLdsPref = 80 * 1024;  // two workgroups per CU
+
+size_t vsize(const sqc::Plan &p) { return p.prec == 1 ? 8 : 16; }
+
+size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
+  const sqc::Geo &g = p.geo;
+  return ((size_t)(txr / 2) * ty * g.cs + g.n[2]) * vsize(p) + sqc::kStageCap * (16 + 2 * 4);
+}
+
+size_t ypass_lds(const sqc::Plan &p, int tx2) {
+  const sqc::Geo &g = p.geo;
+  return ((size_t)tx2 * g.ls2 + g.n[1]) * vsize(p);
+}
+
+int ilog2i(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Whether a spread tile is usable: LDS within `limit`, at least two tiles per
+// axis, and a bin window of at most 128 bins.
+bool tile_fits(const sqc::Plan &p, int txr, int ty, size_t limit) {
+  const sqc::Geo &g = p.geo;
+  const int n1 = g.n[0], n2 = g.n[1], ns = g.ns;
+  if (spread_lds(p, txr, ty) > limit) return false;
+  if (2 * txr > n1 || 2 * ty > n2) return false;
+  const int ntx = (n1 + txr - 1) / txr, nty = (n2 + ty - 1) / ty;
+  const int hx = (int)std::ceil((0.5 * ns + 1.0) / txr) + (n1 % txr ? 1 : 0);
+  const int hy = (int)std::ceil((0.5 * ns + 1.0) / ty) + (n2 % ty ? 1 : 0);
+  return std::min(2 * hx + 1, ntx) * std::min(2 * hy + 1, nty) <= 128;
+}
+
+// Everything that follows from the spread tile and the y-pass column chunk.
+void set_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
+  sqc::Geo &g = p.geo;
+  const int n1 = g.n[0], n2 = g.n[1], ns = g.ns;
+  g.txr = txr;
+  g.ty = ty;
+  p.lds1 = spread_lds(p, txr, ty);
+  g.ltxr = ilog2i(g.txr);
+  g.lty = ilog2i(g.ty);
+  g.ntx = (n1 + g.txr - 1) / g.txr;
+  g.nty = (n2 + g.ty - 1) / g.ty;
+  g.nbins = g.ntx * g.nty;
+  // a stencil that wraps past a narrow last tile can come from one bin further
+  g.hx = (int)std::ceil((0.5 * ns + 1.0) / g.txr) + (n1 % g.txr ? 1 : 0);
+  g.hy = (int)std::ceil((0.5 * ns + 1.0) / g.ty) + (n2 % g.ty ? 1 : 0);
+  g.sx = std::min(g.ntx, (ns - 1 + g.txr - 1) / g.txr + 2);
+  g.sy = std::min(g.nty, (ns - 1 + g.ty - 1) / g.ty + 2);
+  p.tx2 = tx2;
+  g.ltx2 = ilog2i(tx2);
+  p.lds2 = ypass_lds(p, tx2);
+}
+
+// Bin counters of the atom sort (sized by the tile grid); the count array
+// starts at zero and the scan leaves it at zero.
+int alloc_bins(sqc::Plan &p) {
+  const int nbt = p.geo.F * p.geo.nbins;
+  for (int32_t **b : {&p.d_count, &p.d_start, &p.d_cursor}) {
+    if (*b) {
+      (void)hipFree(*b);
+      p.bytes -= (int64_t)sizeof(int32_t) * (p.nbt_alloc + 1);
+      *b = nullptr;
+    }
+  }
+  int rc;
+  if ((rc = alloc(&p.d_count, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
+  if ((rc = alloc(&p.d_start, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
+  if ((rc = alloc(&p.d_cursor, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
+  p.nbt_alloc = nbt;
+  HIP_TRY(hipMemset(p.d_count, 0, sizeof(int32_t) * (nbt + 1)), "hipMemset");
+  return 0;
+}
+
+void drop_graph(sqc::Plan &p) {
+  if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
+  p.gexec = nullptr;
+}
+
 void free_plan(sqc_hip_plan *h) {
   sqc::Plan &p = h->p;
   void *bufs[] = {p.d_pos, p.d_X, p.d_i0, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
@@ -272,6 +358,7 @@ void free_plan(sqc_hip_plan *h) {
     (void)hipEventDestroy(q.b);
   }
   for (hipEvent_t e : p.ev_free) (void)hipEventDestroy(e);
+  drop_graph(p);
   if (p.ev_in) (void)hipEventDestroy(p.ev_in);
   if (p.ev_out) (void)hipEventDestroy(p.ev_out);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
@@ -464,8 +551,8 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
 
   // --- launch shapes ---
   const size_t vs = p.prec == 1 ? 8 : 16;
-  const size_t lds_cap = 160 * 1024;
-  const size_t lds_pref = 80 * 1024;  // two workgroups per CU
+  const size_t lds_cap = kLdsCap;
+  const size_t lds_pref = kLdsPref;
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
   for (int i = 0; i < 3; ++i)
     if (g.n[i] > sqc::kMaxLine)
@@ -479,48 +566,17 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   g.ls2 = n2 + ((1 - n2) % 32 + 32) % 32;
   g.rs1 = n1;
   {
-    static const int cand[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {4, 2}, {4, 1}, {2, 1}};
     int pick = -1;
     for (int pass = 0; pass < 2 && pick < 0; ++pass)
-      for (int c = 0; c < 7; ++c) {
-        const int txr = cand[c][0], ty = cand[c][1];
-        const int cc = (txr / 2) * ty;
-        const size_t lds = ((size_t)cc * g.cs + n3) * vs + sqc::kStageCap * (16 + 2 * 4);
-        if (lds > (pass == 0 ? lds_pref : lds_cap)) continue;
-        if (2 * txr > n1 || 2 * ty > n2) continue;
-        const int ntx = (n1 + txr - 1) / txr, nty = (n2 + ty - 1) / ty;
-        const int hx = (int)std::ceil((0.5 * ns + 1.0) / txr) + (n1 % txr ? 1 : 0);
-        const int hy = (int)std::ceil((0.5 * ns + 1.0) / ty) + (n2 % ty ? 1 : 0);
-        if (std::min(2 * hx + 1, ntx) * std::min(2 * hy + 1, nty) > 128) continue;
-        pick = c;
-        break;
-      }
+      for (int c = 0; c < kNumTiles; ++c)
+        if (tile_fits(p, kTiles[c][0], kTiles[c][1], pass == 0 ? kLdsPref : kLdsCap)) {
+          pick = c;
+          break;
+        }
     if (pick < 0) return fail(SQC_HIP_ERR_MAXNALLOC, "no spread tile fits the LDS");
-    g.txr = cand[pick][0];
-    g.ty = cand[pick][1];
-    p.lds1 = ((size_t)(g.txr / 2) * g.ty * g.cs + n3) * vs + sqc::kStageCap * (16 + 2 * 4);
-  }
-  auto ilog2 = [](int v) {
-    int l = 0;
-    while ((1 << l) < v) ++l;
-    return l;
-  };
-  g.ltxr = ilog2(g.txr);
-  g.lty = ilog2(g.ty);
-  g.ntx = (n1 + g.txr - 1) / g.txr;
-  g.nty = (n2 + g.ty - 1) / g.ty;
-  g.nbins = g.ntx * g.nty;
-  // a stencil that wraps past a narrow last tile can come from one bin further
-  g.hx = (int)std::ceil((0.5 * ns + 1.0) / g.txr) + (n1 % g.txr ? 1 : 0);
-  g.hy = (int)std::ceil((0.5 * ns + 1.0) / g.ty) + (n2 % g.ty ? 1 : 0);
-  g.sx = std::min(g.ntx, (ns - 1 + g.txr - 1) / g.txr + 2);
-  g.sy = std::min(g.nty, (ns - 1 + g.ty - 1) / g.ty + 2);
-  {
     int tx2 = 16;
-    while (tx2 > 1 && ((size_t)tx2 * g.ls2 + n2) * vs > lds_pref) tx2 /= 2;
-    p.tx2 = tx2;
-    g.ltx2 = ilog2(tx2);
-    p.lds2 = ((size_t)tx2 * g.ls2 + n2) * vs;
+    while (tx2 > 1 && ypass_lds(p, tx2) > kLdsPref) tx2 /= 2;
+    set_tiles(p, kTiles[pick][0], kTiles[pick][1], tx2);
   }
   {
     // shell accumulators and the 1 / phihat_1 table follow the rows
@@ -552,9 +608,9 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
 
   // --- device buffers ---
   const int64_t N = cfg->natoms;
-  const int nbt = g.F * g.nbins;
   int rc;
   p.batch_cap = cfg->max_batch > 0 ? cfg->max_batch : 8;
+  p.use_graph = p.batch_cap > 1;
   if ((rc = alloc(&p.d_pos, sizeof(double) * 3 * N * p.batch_cap, p.bytes))) return rc;
   if ((rc = alloc(&p.d_X, sizeof(double) * 3 * N, p.bytes))) return rc;
   if ((rc = alloc(&p.d_i0, sizeof(int4) * N, p.bytes))) return rc;
@@ -563,10 +619,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if ((rc = alloc(&p.d_key, sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = alloc(&p.d_sidx, sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = upload(&p.d_species, p.species_of.data(), sizeof(int32_t) * N, p.bytes))) return rc;
-  if ((rc = alloc(&p.d_count, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
-  if ((rc = alloc(&p.d_start, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
-  if ((rc = alloc(&p.d_cursor, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
-  HIP_TRY(hipMemset(p.d_count, 0, sizeof(int32_t) * (nbt + 1)), "hipMemset");
+  if ((rc = alloc_bins(p))) return rc;
   const size_t w1 = (size_t)g.F * (M3 + 1) * n2 * n1 * vs;
   const size_t w2 = (size_t)g.F * std::max(g.nlines, 1) * n1 * vs;
   if ((rc = alloc((char **)&p.d_W1, w1, p.bytes))) return rc;
@@ -622,6 +675,128 @@ struct OrderScope {
     (void)hipStreamWaitEvent(user, p.ev_out, 0);
   }
 };
+
+// Forward time of one frame (ms, best of two after a warm-up), nothing
+// accumulated.
+int time_forward(sqc::Plan &p, const double *pos_dev, float *best) {
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a), "hipEventCreate");
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
+  }
+  *best = 1e30f;
+  int rc = 0;
+  for (int it = 0; it < 3 && !rc; ++it) {
+    (void)hipEventRecord(a, p.stream);
+    rc = sqc::launch_forward(p, pos_dev, 2, nullptr, nullptr);
+    (void)hipEventRecord(b, p.stream);
+    if (!rc) rc = (int)hipEventSynchronize(b);
+    float ms = 0.f;
+    if (!rc && it > 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess) *best = std::min(*best, ms);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc ? hip_fail((hipError_t)rc, "tuning launch") : 0;
+}
+
+// Re-derive everything that depends on the tiles and resize what they size.
+int apply_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
+  set_tiles(p, txr, ty, tx2);
+  if (int rc = alloc_bins(p)) return rc;
+  if (p.d_slots) {  // sized by the tiles an atom can touch; the adjoint re-allocates it
+    (void)hipFree(p.d_slots);
+    p.bytes -= p.slots_bytes;
+    p.d_slots = nullptr;
+  }
+  if (int e = sqc::kernel_lds_limits(p)) return hip_fail((hipError_t)e, "setting LDS sizes");
+  drop_graph(p);
+  return 0;
+}
+
+// Autotuner (config.tune): on the first frame, time the forward for every
+// spread tile that fits the LDS, then for every y-pass column chunk with the
+// best tile, and keep the fastest.  Nothing is accumulated while tuning.
+int tune(sqc::Plan &p, const double *pos_dev) {
+  p.tuned = 1;
+  if (!p.cfg.tune) return 0;
+  int btxr = p.geo.txr, bty = p.geo.ty, btx2 = p.tx2;
+  float best = 1e30f;
+  for (int c = 0; c < kNumTiles; ++c) {
+    if (!tile_fits(p, kTiles[c][0], kTiles[c][1], kLdsCap)) continue;
+    if (int rc = apply_tiles(p, kTiles[c][0], kTiles[c][1], btx2)) return rc;
+    float ms;
+    if (int rc = time_forward(p, pos_dev, &ms)) return rc;
+    if (ms < best) {
+      best = ms;
+      btxr = kTiles[c][0];
+      bty = kTiles[c][1];
+    }
+  }
+  for (int tx2 = 16; tx2 >= 2; tx2 /= 2) {
+    if (ypass_lds(p, tx2) > kLdsCap || tx2 == btx2) continue;
+    if (int rc = apply_tiles(p, btxr, bty, tx2)) return rc;
+    float ms;
+    if (int rc = time_forward(p, pos_dev, &ms)) return rc;
+    if (ms < best) {
+      best = ms;
+      btx2 = tx2;
+    }
+  }
+  return apply_tiles(p, btxr, bty, btx2);
+}
+
+// Capture batch_cap forward frames reading p.d_pos into one graph.
+int capture_batch(sqc::Plan &p) {
+  const int64_t stride = 3 * p.geo.natoms;
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  int rc = 0;
+  for (int64_t f = 0; f < p.batch_cap && !rc; ++f)
+    rc = sqc::launch_forward(p, p.d_pos + f * stride, 0, nullptr, nullptr);
+  const hipError_t e = hipStreamEndCapture(p.stream, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return hip_fail(rc ? (hipError_t)rc : e, "capturing the frame batch");
+  }
+  const hipError_t ei = hipGraphInstantiate(&p.gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    p.gexec = nullptr;
+    return hip_fail(ei, "hipGraphInstantiate");
+  }
+  return 0;
+}
+
+// Accumulate frames whose positions are in device memory.  Full batches of
+// batch_cap frames replay the captured graph (positions copied into d_pos
+// first unless they are already there); a short tail, and every frame while
+// the stage timers are on, is launched kernel by kernel.
+int run_frames(sqc::Plan &p, const double *pos_dev, int64_t nframes) {
+  const int64_t stride = 3 * p.geo.natoms;
+  if (nframes > 0 && !p.tuned)
+    if (int rc = tune(p, pos_dev)) return rc;
+  for (int64_t f0 = 0; f0 < nframes; f0 += p.batch_cap) {
+    const int64_t nb = std::min<int64_t>(p.batch_cap, nframes - f0);
+    const double *src = pos_dev + f0 * stride;
+    if (p.use_graph && !p.profiling && nb == p.batch_cap) {
+      if (src != p.d_pos)
+        HIP_TRY(hipMemcpyAsync(p.d_pos, src, sizeof(double) * stride * nb,
+                               hipMemcpyDeviceToDevice, p.stream),
+                "stage positions");
+      if (!p.gexec)
+        if (int rc = capture_batch(p)) return rc;
+      HIP_TRY(hipGraphLaunch(p.gexec, p.stream), "hipGraphLaunch");
+    } else {
+      for (int64_t f = 0; f < nb; ++f) {
+        const int rc = sqc::launch_forward(p, src + f * stride, 0, nullptr, nullptr);
+        if (rc) return hip_fail((hipError_t)rc, "forward launch");
+      }
+    }
+    p.nframes += nb;
+  }
+  return 0;
+}
 
 }  // namespace
 
@@ -752,12 +927,7 @@ int sqc_hip_accumulate_device(sqc_hip_plan *plan, const double *pos_device, int6
   if (!pos_device || nframes < 0) return fail(SQC_HIP_ERR_BAD_ARG, "bad positions");
   sqc::Plan &p = plan->p;
   HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
-  const int64_t stride = 3 * p.geo.natoms;
-  for (int64_t f = 0; f < nframes; ++f) {
-    const int rc = sqc::launch_forward(p, pos_device + f * stride, 0, nullptr, nullptr);
-    if (rc) return hip_fail((hipError_t)rc, "forward launch");
-    ++p.nframes;
-  }
+  if (int rc = run_frames(p, pos_device, nframes)) return rc;
   return SQC_HIP_OK;
 }
 
@@ -861,6 +1031,8 @@ int sqc_hip_forward_device(sqc_hip_plan *plan, const double *pos_device, int64_t
   sqc::Plan &p = plan->p;
   HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
   OrderScope scope(p, stream);
+  if (nframes > 0 && !p.tuned)
+    if (int rc = tune(p, pos_device)) return rc;
   const int64_t stride = 3 * p.geo.natoms;
   const size_t vs = p.prec == 1 ? 8 : 16;
   const size_t rstride = (size_t)p.geo.F * p.ncand * vs;
@@ -884,7 +1056,8 @@ int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const v
     return fail(SQC_HIP_ERR_BAD_ARG, "pair gradients need a plan created with partials");
   HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
   if (!p.d_slots) {
-    int rc = alloc(&p.d_slots, sizeof(double) * 3 * p.geo.natoms * p.geo.sx * p.geo.sy, p.bytes);
+    p.slots_bytes = (int64_t)sizeof(double) * 3 * p.geo.natoms * p.geo.sx * p.geo.sy;
+    int rc = alloc(&p.d_slots, (size_t)p.slots_bytes, p.bytes);
     if (rc) return rc;
   }
   OrderScope scope(p, stream);

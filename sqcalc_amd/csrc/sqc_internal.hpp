@@ -105,6 +105,12 @@ struct Plan {
   size_t lds1 = 0, lds2 = 0, lds3 = 0;
   int64_t nframes = 0;
   int64_t batch_cap = 1;
+  int nbt_alloc = 0;           // bins the sort counters were sized for
+  int64_t slots_bytes = 0;     // size of d_slots
+  int tuned = 0;               // autotuner has run (or was not asked for)
+  // batched frames: a captured graph of batch_cap frames read from d_pos
+  hipGraphExec_t gexec = nullptr;
+  int use_graph = 1;
   // device buffers
   double *d_pos = nullptr;
   double *d_X = nullptr;

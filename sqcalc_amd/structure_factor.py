@@ -43,7 +43,8 @@ class HipStructureFactor:
 
     def __init__(self, cell, origin, types, mapping=None, weight="unit", norm="mean",
                  qmin=0.0, qmax=20.0, nq=500, eps=None, precision="double", partials=True,
-                 faber_ziman=False, want_grid=False, device=0, upsampfac=0.0, max_batch=0):
+                 faber_ziman=False, want_grid=False, device=0, upsampfac=0.0, max_batch=0,
+                 tune=False):
         self.lib = _lib.load()
         self.cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
         self.origin = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
@@ -81,7 +82,9 @@ class HipStructureFactor:
         prec = {"double": _lib.F64, "single": _lib.F32}[precision]
         self.precision = precision
         if eps is None:
-            eps = 1e-6 if prec == _lib.F32 else 1e-9
+            # single precision defaults to 1e-5, as the reference does for
+            # --precision single without --eps (app/main.f90:224-230)
+            eps = 1e-5 if prec == _lib.F32 else 1e-9
         cfg = _lib.Config()
         cfg.abi_version = _lib.ABI_VERSION
         cfg.device = device
@@ -103,18 +106,25 @@ class HipStructureFactor:
         cfg.amp_const = self.amp.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         cfg.xray_coef = self.xcoef.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         cfg.max_batch = int(max_batch)
+        cfg.tune = 1 if tune else 0
         plan = ctypes.c_void_p()
         rc = self.lib.sqc_hip_plan_create(ctypes.byref(plan), ctypes.byref(cfg))
         _lib.check(rc, "sqc_hip_plan_create")
         self.plan = plan
-        self.info = _lib.Info()
-        _lib.check(self.lib.sqc_hip_plan_info(self.plan, ctypes.byref(self.info)), "plan_info")
         self.den = np.zeros(self.nq)
         self.shell_count = np.zeros(self.nq, dtype=np.int64)
         _lib.check(self.lib.sqc_hip_shells(self.plan, self.den.ctypes.data,
                                            self.shell_count.ctypes.data), "shells")
 
     # -- plumbing ---------------------------------------------------------
+    @property
+    def info(self):
+        """Plan geometry (queried each time: the autotuner may change the tiles
+        on the first frame)."""
+        info = _lib.Info()
+        _lib.check(self.lib.sqc_hip_plan_info(self.plan, ctypes.byref(info)), "plan_info")
+        return info
+
     def close(self):
         if getattr(self, "plan", None):
             self.lib.sqc_hip_destroy(self.plan)
@@ -176,19 +186,46 @@ class HipStructureFactor:
         return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.STAGES) if cnt[i]}
 
     def stage_bytes(self):
-        """Algorithmic HBM bytes per launch of the three DFT stages: each reads
-        its input once and writes its output once."""
+        """Algorithmic HBM bytes per launch of the transform stages (one frame):
+        each stage reads its input once and writes its output once.
+
+          spread_z      W1 = F (M3+1) n2 n1 b written; per atom the stencil
+                        corner (16 B), its sort index (4 B) and 3 ns kernel
+                        values read
+          dft_y         W1 read, W2 = F L n1 b written (L canonical-half lines)
+          dft_x_bin     W2 read, the line table (20 B per line) and the shell
+                        index of every candidate mode (2 B) read
+          adj_x         rho (F C b, C candidate modes) and the shell table read,
+                        W2 written
+          adj_y         W2 read, W1 written
+          adj_z_gather  W1 and per atom 16 + 4 B plus 6 ns kernel values read,
+                        3 doubles per (atom, tile) slot written
+        with b = 8 (FP32 complex) or 16 (FP64 complex).
+        """
         i = self.info
         vs = 8 if self.precision == "single" else 16
+        st = vs // 2
         n1, n2 = i.fine[0], i.fine[1]
         M3 = (i.modes[2] - 1) // 2
         F = i.nfields
+        ns = i.nspread
         w1 = F * (M3 + 1) * n2 * n1 * vs
         w2 = F * i.nlines * n1 * vs
-        atoms = self.natoms * (3 * 8 + 4 + 4)
+        C = i.rho_len
+        atoms = self.natoms * (16 + 4 + 3 * ns * st)
+        slots = self.natoms * self._slots_per_atom() * 3 * 8
         return {"spread_z": w1 + atoms, "dft_y": w1 + w2,
-                "dft_x_bin": w2 + 2 * i.rho_len, "adj_x": w2 + F * vs * i.rho_len,
-                "adj_y": w1 + w2, "adj_z_gather": w1 + atoms}
+                "dft_x_bin": w2 + 20 * i.nlines + 2 * C,
+                "adj_x": F * C * vs + 2 * C + 20 * i.nlines + w2, "adj_y": w1 + w2,
+                "adj_z_gather": w1 + atoms + self.natoms * 3 * ns * st + slots}
+
+    def _slots_per_atom(self):
+        t = self.info.tile
+        ns = self.info.nspread
+        fine = self.info.fine
+        sx = min(-(-fine[0] // t[0]), (ns - 1 + t[0] - 1) // t[0] + 2)
+        sy = min(-(-fine[1] // t[1]), (ns - 1 + t[1] - 1) // t[1] + 2)
+        return sx * sy
 
     # -- output (sf_shell_value, sf_partial_value, sf_write_results) ---------
     def table(self):
