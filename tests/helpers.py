@@ -90,3 +90,41 @@ def candidate_layout(md):
                 slots.append((h1, h2, h3) if (h1, h2, h3) in kept and canon(h1, h2, h3)
                              else None)
     return slots
+
+
+def write_dump(path, frames, types, lo, hi, tilt=(0.0, 0.0, 0.0), scaled=False, ids=None,
+               natoms_override=None):
+    """LAMMPS text dump of Cartesian frames (N, 3): `x y z`, or `xs ys zs`
+    when scaled (fractional coordinates of the restricted triclinic cell).
+    The format written by test/gen_dump.py:134-142 (%.10f columns)."""
+    cell, origin = O.cell_from_bounds(lo, hi, tilt)
+    inv = np.linalg.inv(cell)
+    n = len(types)
+    triclinic = any(abs(t) > 0 for t in tilt)
+    with open(path, "w") as fh:
+        for k, pos in enumerate(frames):
+            fh.write("ITEM: TIMESTEP\n%d\n" % (100 * k))
+            fh.write("ITEM: NUMBER OF ATOMS\n%d\n" % (natoms_override or n))
+            if triclinic:
+                fh.write("ITEM: BOX BOUNDS xy xz yz pp pp pp\n")
+                for i in range(3):
+                    fh.write("%.10f %.10f %.10f\n" % (lo[i], hi[i], tilt[i]))
+            else:
+                fh.write("ITEM: BOX BOUNDS pp pp pp\n")
+                for i in range(3):
+                    fh.write("%.10f %.10f\n" % (lo[i], hi[i]))
+            cols = "xs ys zs" if scaled else "x y z"
+            fh.write("ITEM: ATOMS id type %s\n" % cols)
+            order = ids[k] if ids is not None else np.arange(1, n + 1)
+            for j in range(n):
+                r = np.asarray(pos[j], dtype=np.float64)
+                v = inv @ (r - origin) if scaled else r
+                fh.write("%d %d %.10f %.10f %.10f\n" % (order[j], types[j], v[0], v[1], v[2]))
+
+
+def read_table(text):
+    """(header, rows) of a `# q S(q) ...` table."""
+    lines = [l for l in text.splitlines() if l.strip()]
+    header = lines[0]
+    rows = np.array([[float(x) for x in l.split()] for l in lines[1:]])
+    return header, rows

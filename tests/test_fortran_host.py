@@ -1,0 +1,178 @@
+"""Fortran host (sqcalc_amd/fortran): options, exit codes and scattering
+tables on the CPU; the S(q) tables it writes on the GPU.
+
+The exit codes and messages follow `sqcalc static` (app/main.f90: 2 bad
+options, 4 unreadable dump, 5 no frame, 6 set-up, 7 malformed frame, 8 atom
+count change, 9 box change, 24 id order change).  Dumps are written from the
+golden fixtures, the parity reference is the golden table (FP64 within
+1e-9) and the Python host on the same plan settings.
+"""
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import golden_cases, load_golden, read_table, rel_dev, write_dump
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "sqcalc_amd", "fortran", "sqcalc_hip")
+HAVE_FLANG = shutil.which("amdflang") or os.path.exists("/opt/rocm/bin/amdflang")
+
+
+def run(args, **kw):
+    return subprocess.run([EXE] + list(args), capture_output=True, text=True, timeout=300, **kw)
+
+
+@pytest.fixture(scope="module")
+def exe():
+    if not os.path.exists(EXE):
+        if HAVE_FLANG:
+            pytest.fail("sqcalc_hip is not built (run __graft_entry__.build())")
+        pytest.skip("no Fortran compiler and no sqcalc_hip binary")
+    return EXE
+
+
+def test_element_tables_match_python(exe):
+    from sqcalc_amd import elements
+    r = run(["elements"])
+    assert r.returncode == 0, r.stderr
+    seen_n, seen_x = set(), set()
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if f[0] == "neutron":
+            assert float(f[2]) == pytest.approx(elements.NEUTRON_B[f[1]], rel=1e-15)
+            seen_n.add(f[1])
+        elif f[0] == "xray":
+            np.testing.assert_allclose([float(x) for x in f[2:]], elements.XRAY[f[1]], rtol=1e-15)
+            seen_x.add(f[1])
+    assert seen_n == set(elements.NEUTRON_B) and seen_x == set(elements.XRAY)
+
+
+@pytest.mark.parametrize("args,code,text", [
+    ([], 2, "subcommand"),
+    (["dynamic"], 2, "subcommand"),
+    (["static"], 2, "missing input"),
+    (["static", "-i", "x.dump", "--bogus"], 2, "unknown option"),
+    (["static", "-i", "x.dump", "--device", "cpu"], 2, "GPU path only"),
+    (["static", "-i", "x.dump", "-w", "photon"], 2, "unknown weight"),
+    (["static", "-i", "x.dump", "-w", "neutron"], 2, "mapping"),
+    (["static", "-i", "x.dump", "--qmin", "5", "--qmax", "4"], 2, "larger than qmin"),
+    (["static", "-i", "x.dump", "--precision", "half"], 2, "unknown precision"),
+    (["static", "-i", "/nonexistent/x.dump"], 4, "cannot open"),
+])
+def test_option_errors(exe, args, code, text):
+    r = run(args)
+    assert r.returncode == code, (r.returncode, r.stderr)
+    assert text in r.stderr
+
+
+def test_empty_dump(exe, tmp_path):
+    p = tmp_path / "empty.dump"
+    p.write_text("")
+    r = run(["static", "-i", str(p)])
+    assert r.returncode == 5 and "no complete frame" in r.stderr
+
+
+def test_setup_without_device(exe, tmp_path):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    d = load_golden("gas_unit_self")
+    p = tmp_path / "g.dump"
+    write_dump(p, d["frames"], d["types"], d["lo"], d["hi"], d["tilt"])
+    r = run(["static", "-i", str(p), "--quiet"])
+    assert r.returncode == 6, r.stderr
+    assert "device" in r.stderr.lower()
+
+
+# --------------------------------------------------------------------- GPU
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases())
+def test_golden_tables_through_fortran(exe, hip_device, tmp_path, name):
+    d = load_golden(name)
+    scaled = name.startswith("tri")
+    p = tmp_path / (name + ".dump")
+    write_dump(p, d["frames"], d["types"], d["lo"], d["hi"], d["tilt"], scaled=scaled)
+    out = tmp_path / "sq.txt"
+    print("case fortran golden", name, flush=True)
+    r = run(["static", "-i", str(p), "-m", d["mapping"], "-w", d["weight"], "--norm", d["norm"],
+             "--qmin", repr(d["qmin"]), "--qmax", repr(d["qmax"]), "--nq", str(d["nq"]),
+             "--eps", "1e-10", "--no-partials", "--quiet", str(out)])
+    assert r.returncode == 0, r.stderr
+    header, rows = read_table(out.read_text())
+    assert header.strip() == "# q S(q)"
+    np.testing.assert_allclose(rows[:, 0], np.round(np.array(d["q"]), 6), atol=1e-12)
+    dev = rel_dev(np.array(d["S"]), rows[:, 1])
+    print("  max deviation %.2e" % dev)
+    assert dev < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,fz", [("double", False), ("single", True)])
+def test_partials_match_python_host(exe, hip_device, tmp_path, precision, fz):
+    from sqcalc_amd import HipStructureFactor
+    from helpers import random_case
+    cell, origin, types, frames = random_case(natoms=600, length=18.0, nframes=3, seed=41,
+                                              ntypes=3)
+    lo, hi = [0.0] * 3, [18.0] * 3
+    p = tmp_path / "r.dump"
+    write_dump(p, frames, types, lo, hi)
+    out = tmp_path / "sq.txt"
+    grid = tmp_path / "grid.txt"
+    args = ["static", "-i", str(p), "-m", "1:Si,2:O,3:Na", "-w", "xray", "--qmax", "5",
+            "--nq", "40", "--precision", precision, "--grid", str(grid), "--quiet", str(out)]
+    if fz:
+        args.insert(1, "-fz")
+    print("case fortran partials", precision, fz, flush=True)
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    header, rows = read_table(out.read_text())
+    # the dump rounds positions to 1e-10
+    frames_r = np.round(frames, 10)
+    sf = HipStructureFactor(cell, origin, types, {1: "Si", 2: "O", 3: "Na"}, weight="xray",
+                            qmax=5.0, nq=40, precision=precision, faber_ziman=fz,
+                            want_grid=True)
+    sf.accumulate(frames_r)
+    ref_header = sf.format_table().splitlines()[0]
+    assert header == ref_header
+    tol = 1e-12 if precision == "double" else 1e-6
+    assert rel_dev(sf.table(), rows) < 1e-11 + tol
+    qv, gs = sf.grid_table()
+    g = np.loadtxt(grid)
+    np.testing.assert_allclose(g[:, :3], np.round(qv, 8), atol=1e-12)
+    assert rel_dev(gs, g[:, 3]) < 1e-11 + tol
+    sf.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("change,code", [("natoms", 8), ("box", 9), ("order", 24),
+                                         ("truncated", 7)])
+def test_frame_consistency_errors(exe, hip_device, tmp_path, change, code):
+    d = load_golden("gas_unit_self")
+    frames, types = d["frames"], d["types"]
+    p = tmp_path / "bad.dump"
+    write_dump(p, frames[:1], types, d["lo"], d["hi"], d["tilt"])
+    n = len(types)
+    second = tmp_path / "second.dump"
+    if change == "natoms":
+        write_dump(second, frames[1:2, :-1], types[:-1], d["lo"], d["hi"], d["tilt"])
+    elif change == "box":
+        hi = [h + 1.0 for h in d["hi"]]
+        write_dump(second, frames[1:2], types, d["lo"], hi, d["tilt"])
+    elif change == "order":
+        ids = [np.concatenate([[2, 1], np.arange(3, n + 1)])]
+        write_dump(second, frames[1:2], types, d["lo"], d["hi"], d["tilt"], ids=ids)
+    else:
+        write_dump(second, frames[1:2], types, d["lo"], d["hi"], d["tilt"])
+        text = second.read_text().splitlines()[:-3]
+        second.write_text("\n".join(text) + "\n")
+    with open(p, "a") as fh:
+        fh.write(second.read_text())
+    print("case fortran frame error", change, flush=True)
+    r = run(["static", "-i", str(p), "--quiet", str(tmp_path / "o.txt")])
+    assert r.returncode == code, (r.returncode, r.stderr)
