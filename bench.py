@@ -83,11 +83,15 @@ def main():
                                   precision=precision, **kw)
 
     # ---- FP32-spread forward (headline): one graph replay per step ----
-    sf = make("single", max_batch=fps)
+    sf = make("single", max_batch=fps, tune=True)
     info = sf.info
     print("# fp32 plan: m=%s fine=%s ns=%d sigma=%.2f K=%d lines=%d tiles=%s bytes=%.2f GB"
           % (list(info.modes), list(info.fine), info.nspread, info.upsampfac, info.nmodes,
              info.nlines, list(info.tile), info.device_bytes / 1e9), file=sys.stderr, flush=True)
+    sf.accumulate_device(pos[0:1])  # the tuner times the tiles on the first frame
+    info = sf.info
+    print("# tuned tiles %s" % list(info.tile), file=sys.stderr, flush=True)
+    sf.reset()
     for w in range(args.warmup):
         sf.accumulate_device(pos[(w % nbatch) * fps:(w % nbatch + 1) * fps])
     torch.cuda.synchronize()
@@ -132,9 +136,10 @@ def main():
     # ---- plain FP64 forward (sigma 2) ----
     fp64 = None
     if not args.no_fp64:
-        sd = make("double")
+        sd = make("double", tune=True, max_batch=args.fp64_frames)
         nf = min(args.fp64_frames, pos.shape[0])
-        sd.accumulate_device(pos[0:1])
+        sd.accumulate_device(pos[0:1])  # tunes on the first frame
+        sd.accumulate_device(pos[0:nf])  # warm-up of the graph
         sd.fetch()
         t0 = time.perf_counter()
         sd.accumulate_device(pos[0:nf])
@@ -147,7 +152,7 @@ def main():
     adj = None
     if not args.no_adjoint:
         from sqcalc_amd.autograd import sq_matching_loss
-        sa = make("single", partials=False)
+        sa = make("single", partials=False, tune=True)
         tgt_sf = make("single", partials=False)
         tgt_sf.accumulate_device(pos[-1:])
         num, _, _, _ = tgt_sf.fetch()

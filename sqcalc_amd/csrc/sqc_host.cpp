@@ -259,10 +259,15 @@ double amplitude(const sqc::Geo &g, int sp, double q) {
 // Spread tiles (txr real x-columns, ty y-columns), largest first.  txr <= 16
 // and ty <= 2 keep a stencil's xy footprint inside a tile at 64 cells or
 // fewer, which the spreader's per-lane xy table relies on.
-constexpr int kTiles[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {4, 2}, {4, 1}, {2, 1}};
+constexpr int kTiles[][2] = {{16, 2}, {8, 2}, {16, 1}, {4, 2}, {8, 1}, {4, 1}, {2, 1}};
 constexpr int kNumTiles = 7;
 constexpr size_t kLdsCap = 160 * 1024;
-constexpr size_t kLdsPref = 80 * 1024;  // two workgroups per CU
+// Default LDS budgets when the tuner is off.  FP32: at least three spread and
+// x-pass workgroups and four y-pass workgroups resident per CU, so that the
+// global loads of some overlap the DFTs of others.  FP64 lines are twice as
+// large; there two spread/x-pass workgroups and a 64 KB y pass do better.
+size_t lds_pref(const sqc::Plan &p) { return p.prec == 1 ? 56 * 1024 : 80 * 1024; }
+size_t lds_pref_y(const sqc::Plan &p) { return p.prec == 1 ? 40 * 1024 : 64 * 1024; }
 
 size_t vsize(const sqc::Plan &p) { return p.prec == 1 ? 8 : 16; }
 
@@ -334,6 +339,38 @@ int alloc_bins(sqc::Plan &p) {
   if ((rc = alloc(&p.d_cursor, sizeof(int32_t) * (nbt + 1), p.bytes))) return rc;
   p.nbt_alloc = nbt;
   HIP_TRY(hipMemset(p.d_count, 0, sizeof(int32_t) * (nbt + 1)), "hipMemset");
+  return 0;
+}
+
+// x pass: lb lines x F fields of n1 points, the twiddles, the shell
+// accumulators and the 1 / phihat_1 table.
+size_t xpass_lds(const sqc::Plan &p, int lb) {
+  const sqc::Geo &g = p.geo;
+  const size_t accb = ((size_t)g.ncomp * g.nq + g.M[0] + 1) * sizeof(double);
+  return ((size_t)lb * g.F * g.rs1 + g.n[0]) * vsize(p) + accb;
+}
+
+// Lines per group and the persistent grid: as many workgroups as the CUs
+// hold at once (LDS-limited, at most 8 of 4 waves per CU).
+void set_xpass(sqc::Plan &p, int lb) {
+  p.lb = lb;
+  p.lds3 = xpass_lds(p, lb);
+  const int resident = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsCap / p.lds3));
+  const int ngroups = (p.geo.nlines + lb - 1) / lb;
+  p.nblk3 = std::max(1, std::min(ngroups, resident * p.ncu));
+}
+
+// Per-workgroup shell rows of the x pass (one row per persistent workgroup).
+int alloc_wgpart(sqc::Plan &p) {
+  const size_t n = (size_t)p.geo.ncomp * p.geo.nq * p.nblk3;
+  if (p.d_wgpart && p.wgpart_rows == p.nblk3) return 0;
+  if (p.d_wgpart) {
+    (void)hipFree(p.d_wgpart);
+    p.bytes -= (int64_t)sizeof(double) * p.geo.ncomp * p.geo.nq * p.wgpart_rows;
+    p.d_wgpart = nullptr;
+  }
+  if (int rc = alloc(&p.d_wgpart, sizeof(double) * n, p.bytes)) return rc;
+  p.wgpart_rows = p.nblk3;
   return 0;
 }
 
@@ -552,7 +589,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   // --- launch shapes ---
   const size_t vs = p.prec == 1 ? 8 : 16;
   const size_t lds_cap = kLdsCap;
-  const size_t lds_pref = kLdsPref;
+  const size_t lds_pref = ::lds_pref(p);
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
   for (int i = 0; i < 3; ++i)
     if (g.n[i] > sqc::kMaxLine)
@@ -569,26 +606,19 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
     int pick = -1;
     for (int pass = 0; pass < 2 && pick < 0; ++pass)
       for (int c = 0; c < kNumTiles; ++c)
-        if (tile_fits(p, kTiles[c][0], kTiles[c][1], pass == 0 ? kLdsPref : kLdsCap)) {
+        if (tile_fits(p, kTiles[c][0], kTiles[c][1], pass == 0 ? lds_pref : kLdsCap)) {
           pick = c;
           break;
         }
     if (pick < 0) return fail(SQC_HIP_ERR_MAXNALLOC, "no spread tile fits the LDS");
     int tx2 = 16;
-    while (tx2 > 1 && ypass_lds(p, tx2) > kLdsPref) tx2 /= 2;
+    while (tx2 > 1 && ypass_lds(p, tx2) > lds_pref_y(p)) tx2 /= 2;
     set_tiles(p, kTiles[pick][0], kTiles[pick][1], tx2);
   }
-  {
-    // shell accumulators and the 1 / phihat_1 table follow the rows
-    const size_t accb = ((size_t)g.ncomp * g.nq + g.M[0] + 1) * sizeof(double);
-    int lb = sqc::kMaxLb;
-    auto lds3 = [&](int l) { return ((size_t)l * g.F * g.rs1 + n1) * vs + accb; };
-    while (lb > 1 && lds3(lb) > lds_pref) --lb;
-    if (lds3(lb) > lds_cap)
-      return fail(SQC_HIP_ERR_MAXNALLOC, "x line x species too large for the LDS DFT");
-    p.lb = lb;
-    p.lds3 = lds3(lb);
-  }
+  int lb0 = sqc::kMaxLb;
+  while (lb0 > 1 && xpass_lds(p, lb0) > lds_pref) --lb0;
+  if (xpass_lds(p, lb0) > lds_cap)
+    return fail(SQC_HIP_ERR_MAXNALLOC, "x line x species too large for the LDS DFT");
   int dev_count = 0;
   HIP_TRY(hipGetDeviceCount(&dev_count), "hipGetDeviceCount");
   if (cfg->device < 0 || cfg->device >= dev_count)
@@ -599,8 +629,8 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
-  const int ngroups = (g.nlines + p.lb - 1) / p.lb;
-  p.nblk3 = std::max(1, std::min(ngroups, 2 * prop.multiProcessorCount));
+  p.ncu = prop.multiProcessorCount;
+  set_xpass(p, lb0);
   HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking), "hipStreamCreate");
   p.stream = h->own_stream;
   HIP_TRY(hipEventCreateWithFlags(&p.ev_in, hipEventDisableTiming), "hipEventCreate");
@@ -642,7 +672,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
     if ((rc = upload(&p.d_cnt, cnt.data(), cnt.size() * 8, p.bytes))) return rc;
   }
   const size_t ntot = (size_t)g.ncomp * g.nq;
-  if ((rc = alloc(&p.d_wgpart, sizeof(double) * ntot * p.nblk3, p.bytes))) return rc;
+  if ((rc = alloc_wgpart(p))) return rc;
   if ((rc = alloc(&p.d_acc, sizeof(double) * ntot * 2, p.bytes))) return rc;
   HIP_TRY(hipMemset(p.d_acc, 0, sizeof(double) * ntot * 2), "hipMemset");
   if (cfg->want_grid) {
@@ -701,9 +731,11 @@ int time_forward(sqc::Plan &p, const double *pos_dev, float *best) {
 }
 
 // Re-derive everything that depends on the tiles and resize what they size.
-int apply_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
+int apply_tiles(sqc::Plan &p, int txr, int ty, int tx2, int lb) {
   set_tiles(p, txr, ty, tx2);
+  set_xpass(p, lb);
   if (int rc = alloc_bins(p)) return rc;
+  if (int rc = alloc_wgpart(p)) return rc;
   if (p.d_slots) {  // sized by the tiles an atom can touch; the adjoint re-allocates it
     (void)hipFree(p.d_slots);
     p.bytes -= p.slots_bytes;
@@ -715,16 +747,17 @@ int apply_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
 }
 
 // Autotuner (config.tune): on the first frame, time the forward for every
-// spread tile that fits the LDS, then for every y-pass column chunk with the
-// best tile, and keep the fastest.  Nothing is accumulated while tuning.
+// spread tile that fits the LDS, then for every y-pass column chunk and every
+// x-pass line-group size with the best choices so far, and keep the fastest.
+// Nothing is accumulated while tuning.
 int tune(sqc::Plan &p, const double *pos_dev) {
   p.tuned = 1;
   if (!p.cfg.tune) return 0;
-  int btxr = p.geo.txr, bty = p.geo.ty, btx2 = p.tx2;
+  int btxr = p.geo.txr, bty = p.geo.ty, btx2 = p.tx2, blb = p.lb;
   float best = 1e30f;
   for (int c = 0; c < kNumTiles; ++c) {
     if (!tile_fits(p, kTiles[c][0], kTiles[c][1], kLdsCap)) continue;
-    if (int rc = apply_tiles(p, kTiles[c][0], kTiles[c][1], btx2)) return rc;
+    if (int rc = apply_tiles(p, kTiles[c][0], kTiles[c][1], btx2, blb)) return rc;
     float ms;
     if (int rc = time_forward(p, pos_dev, &ms)) return rc;
     if (ms < best) {
@@ -735,7 +768,7 @@ int tune(sqc::Plan &p, const double *pos_dev) {
   }
   for (int tx2 = 16; tx2 >= 2; tx2 /= 2) {
     if (ypass_lds(p, tx2) > kLdsCap || tx2 == btx2) continue;
-    if (int rc = apply_tiles(p, btxr, bty, tx2)) return rc;
+    if (int rc = apply_tiles(p, btxr, bty, tx2, blb)) return rc;
     float ms;
     if (int rc = time_forward(p, pos_dev, &ms)) return rc;
     if (ms < best) {
@@ -743,7 +776,17 @@ int tune(sqc::Plan &p, const double *pos_dev) {
       btx2 = tx2;
     }
   }
-  return apply_tiles(p, btxr, bty, btx2);
+  for (int lb = sqc::kMaxLb; lb >= 1; lb /= 2) {
+    if (xpass_lds(p, lb) > kLdsCap || lb == blb) continue;
+    if (int rc = apply_tiles(p, btxr, bty, btx2, lb)) return rc;
+    float ms;
+    if (int rc = time_forward(p, pos_dev, &ms)) return rc;
+    if (ms < best) {
+      best = ms;
+      blb = lb;
+    }
+  }
+  return apply_tiles(p, btxr, bty, btx2, blb);
 }
 
 // Capture batch_cap forward frames reading p.d_pos into one graph.

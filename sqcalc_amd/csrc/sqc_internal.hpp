@@ -102,6 +102,8 @@ struct Plan {
   int tx2 = 16;               // y-pass x-columns per workgroup
   int lb = 8;                 // x-pass lines per workgroup step
   int nblk3 = 0;              // x-pass persistent workgroups
+  int wgpart_rows = 0;        // rows d_wgpart was sized for
+  int ncu = 1;                // compute units of the device
   size_t lds1 = 0, lds2 = 0, lds3 = 0;
   int64_t nframes = 0;
   int64_t batch_cap = 1;

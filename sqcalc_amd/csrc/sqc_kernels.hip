@@ -386,14 +386,28 @@ __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__r
   atomicAdd(&count[k], 1);
 }
 
+// Exclusive scan of the bin counts by one workgroup: thread t owns the
+// contiguous chunk [t c, (t+1) c) with c a multiple of 4, read as int4 (all of
+// a thread's loads are independent, so they are in flight together); the
+// chunk sums are scanned across the block, then every bin gets its start, the
+// scatter cursor, and its count cleared for the next frame.
 __global__ void __launch_bounds__(1024) k_scan(int32_t *__restrict__ count, int32_t *__restrict__ start,
                                                int32_t *__restrict__ cursor, int nb) {
   __shared__ int part[1024];
   const int tid = threadIdx.x;
-  const int chunk = (nb + 1023) / 1024;
+  const int chunk = (((nb + 1023) / 1024) + 3) & ~3;
   const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
   int s = 0;
-  for (int b = b0; b < b1; ++b) s += count[b];
+  {
+    const int nv = (b1 - b0) >> 2;
+    const int4 *c4 = reinterpret_cast<const int4 *>(count + b0);
+#pragma unroll 8
+    for (int v = 0; v < nv; ++v) {
+      const int4 c = c4[v];
+      s += c.x + c.y + c.z + c.w;
+    }
+    for (int b = b0 + 4 * nv; b < b1; ++b) s += count[b];
+  }
   part[tid] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -403,11 +417,27 @@ __global__ void __launch_bounds__(1024) k_scan(int32_t *__restrict__ count, int3
     __syncthreads();
   }
   int run = tid ? part[tid - 1] : 0;
-  for (int b = b0; b < b1; ++b) {
-    start[b] = run;
-    cursor[b] = run;
-    run += count[b];
-    count[b] = 0;  // ready for the next frame
+  {
+    const int nv = (b1 - b0) >> 2;
+    int4 *c4 = reinterpret_cast<int4 *>(count + b0);
+    int4 *s4 = reinterpret_cast<int4 *>(start + b0);
+    int4 *u4 = reinterpret_cast<int4 *>(cursor + b0);
+#pragma unroll 4
+    for (int v = 0; v < nv; ++v) {
+      const int4 c = c4[v];
+      const int4 o = int4{run, run + c.x, run + c.x + c.y, run + c.x + c.y + c.z};
+      s4[v] = o;
+      u4[v] = o;
+      c4[v] = int4{0, 0, 0, 0};  // ready for the next frame
+      run = o.w + c.w;
+    }
+    for (int b = b0 + 4 * nv; b < b1; ++b) {
+      const int c = count[b];
+      start[b] = run;
+      cursor[b] = run;
+      count[b] = 0;
+      run += c;
+    }
   }
   if (tid == 1023) start[nb] = part[1023];
 }
