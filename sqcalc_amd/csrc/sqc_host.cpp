@@ -276,9 +276,18 @@ size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
   return ((size_t)(txr / 2) * ty * g.cs + g.n[2]) * vsize(p) + sqc::kStageCap * (16 + 2 * 4);
 }
 
+// y-line stride: the transposed copies put lane (line, y) at line * ls2 + y
+// (line fastest, tx2 lines); with ls2 = 32 / tx2 mod 32 elements (FP32) or
+// 16 / tx2 mod 16 (FP64) the lanes of a half wave hit distinct banks.
+int ystride(const sqc::Plan &p, int tx2) {
+  const int n2 = p.geo.n[1];
+  const int period = p.prec == 1 ? 32 : 16;
+  const int want = (period / std::min(tx2, period)) % period;
+  return n2 + ((want - n2) % period + period) % period;
+}
+
 size_t ypass_lds(const sqc::Plan &p, int tx2) {
-  const sqc::Geo &g = p.geo;
-  return ((size_t)tx2 * g.ls2 + g.n[1]) * vsize(p);
+  return ((size_t)tx2 * ystride(p, tx2) + p.geo.n[1]) * vsize(p);
 }
 
 int ilog2i(int v) {
@@ -319,6 +328,7 @@ void set_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
   g.sy = std::min(g.nty, (ns - 1 + g.ty - 1) / g.ty + 2);
   p.tx2 = tx2;
   g.ltx2 = ilog2i(tx2);
+  g.ls2 = ystride(p, tx2);
   p.lds2 = ypass_lds(p, tx2);
 }
 
@@ -596,11 +606,9 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
       return fail(SQC_HIP_ERR_MAXNALLOC, "fine grid of " + std::to_string(g.n[i]) +
                                              " points per axis exceeds the LDS DFT limit of " +
                                              std::to_string(sqc::kMaxLine));
-  // LDS line strides: a z column stride of 24 words mod 64 spreads the
-  // neighbouring columns of a spread update over the banks; a y line stride
-  // of 2 words mod 64 does the same for the transposed loads
+  // LDS z column stride of the spread tile: 24 words mod 64 spreads the
+  // neighbouring columns of a spread update over the banks
   g.cs = n3 + ((12 - n3) % 32 + 32) % 32;
-  g.ls2 = n2 + ((1 - n2) % 32 + 32) % 32;
   g.rs1 = n1;
   {
     int pick = -1;
@@ -630,6 +638,15 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
   p.ncu = prop.multiProcessorCount;
+  {
+    int nx = 1;
+    if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, p.device) != hipSuccess ||
+        nx < 1) {
+      (void)hipGetLastError();
+      nx = 1;
+    }
+    g.nxcd = nx;
+  }
   set_xpass(p, lb0);
   HIP_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking), "hipStreamCreate");
   p.stream = h->own_stream;

@@ -73,6 +73,7 @@ struct Geo {
   int64_t natoms;
   double amp[kMaxSpecies];
   double xcoef[kMaxSpecies][9];
+  int nxcd;         // XCDs the workgroups are dealt to
 };
 
 struct Plan {

@@ -340,6 +340,17 @@ __device__ __forceinline__ void batched_copy(int total, Load load, Store store) 
   }
 }
 
+// The hardware deals consecutive workgroups to the XCDs in turn.  Renumber
+// them so that every XCD works through one contiguous range of logical
+// blocks: neighbouring x chunks of the y pass, which read the two halves of
+// the same cache lines, then share one L2 (without it the y pass fetched
+// 2.5x its bytes from HBM).  A bijection of [0, nb) for any nb and XCD count.
+__device__ __forceinline__ int xcd_block(int i, int nb, int nxcd) {
+  const int xcd = i % nxcd, k = i / nxcd;
+  const int q = nb / nxcd, r = nb % nxcd;
+  return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
 // q = c / d for small non-negative c, d through a float reciprocal; the 0.5
 // offset keeps c / d at least 1 / (2 d) away from an integer.
 __device__ __forceinline__ int small_div(int c, float inv_d) { return (int)(((float)c + 0.5f) * inv_d); }
@@ -676,7 +687,10 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   int4 *rec = reinterpret_cast<int4 *>(tw + n3);
   unsigned short *slist = reinterpret_cast<unsigned short *>(rec + kStageCap);
   T *tflat = reinterpret_cast<T *>(tile);
-  const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx, f = blockIdx.y;
+  // grid (tiles, fields) in hardware order: measured faster here than the
+  // XCD-contiguous renumbering the y pass uses
+  const int f = blockIdx.y, tb = blockIdx.x;
+  const int tx = tb % g.ntx, ty = tb / g.ntx;
   const int x0 = tx * g.txr, y0 = ty * g.ty;
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -739,25 +753,24 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   __syncthreads();
   for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, tw, lane);
   __syncthreads();
-  // split the packed pairs and write h3 = 0..M3
+  // split the packed pairs and write h3 = 0..M3.  txr * ty is a power of two
+  // that divides NT, so every thread keeps one (xl, yl) column and steps h3.
   const int M3 = g.M[2];
-  // txr and ty are powers of two: the tile index splits with shifts
-  const int per = g.ty << g.ltxr;
-  for (int idx = tid; idx < (M3 + 1) * per; idx += NT) {
-    const int xl = idx & (g.txr - 1);
-    const int r = idx >> g.ltxr;
-    const int yl = r & (g.ty - 1);
-    const int k3 = r >> g.lty;
-    if (xl >= txw || yl >= tyw) continue;
-    const int col = yl * ccx + (xl >> 1);
-    const V Z = tile[(size_t)col * cs + (k3)];
-    const V Zm = tile[(size_t)col * cs + ((n3 - k3) % n3)];
-    V o;
-    if ((xl & 1) == 0)
-      o = V{T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y - Zm.y)};
-    else
-      o = V{T(0.5) * (Z.y + Zm.y), T(-0.5) * (Z.x - Zm.x)};
-    W1[(((size_t)f * (M3 + 1) + k3) * n2 + (y0 + yl)) * n1 + x0 + xl] = o;
+  const int lper = g.ltxr + g.lty;
+  const int xl = tid & (g.txr - 1), yl = (tid >> g.ltxr) & (g.ty - 1);
+  if (xl >= txw || yl >= tyw) return;
+  const int kstep = NT >> lper;
+  const V *colp = tile + (yl * ccx + (xl >> 1)) * cs;
+  const size_t plane = (size_t)n2 * n1;
+  V *out = W1 + ((size_t)f * (M3 + 1) * n2 + (y0 + yl)) * n1 + x0 + xl;
+  const T sgn = (xl & 1) ? T(-0.5) : T(0.5);
+  for (int k3 = tid >> lper; k3 <= M3; k3 += kstep) {
+    const V Z = colp[k3];
+    const V Zm = colp[k3 ? n3 - k3 : 0];
+    // even column: (Z + conj Zm) / 2; odd column: (Z - conj Zm) / (2i)
+    const V o = (xl & 1) ? V{T(0.5) * (Z.y + Zm.y), sgn * (Z.x - Zm.x)}
+                         : V{sgn * (Z.x + Zm.x), sgn * (Z.y - Zm.y)};
+    out[(size_t)k3 * plane] = o;
   }
 }
 
@@ -778,7 +791,9 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
   const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
   V *lines = reinterpret_cast<V *>(smem);
   V *tw = lines + (size_t)tx2 * ls;
-  const int x0 = blockIdx.x * tx2, k3 = blockIdx.y, f = blockIdx.z;
+  const int nbx = (g.n[0] + tx2 - 1) / tx2;
+  const int blk = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
+  const int x0 = (blk % nbx) * tx2, k3 = (blk / nbx) % (g.M[2] + 1), f = blk / (nbx * (g.M[2] + 1));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lo = lo2[k3], hi = hi2[k3];
   if (hi < lo) return;
@@ -1103,7 +1118,9 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
   const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
   V *lines = reinterpret_cast<V *>(smem);
   V *tw = lines + (size_t)tx2 * ls;
-  const int x0 = blockIdx.x * tx2, k3 = blockIdx.y, cf = blockIdx.z;
+  const int nbx = (g.n[0] + tx2 - 1) / tx2;
+  const int blk = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
+  const int x0 = (blk % nbx) * tx2, k3 = (blk / nbx) % (g.M[2] + 1), cf = blk / (nbx * (g.M[2] + 1));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lo = lo2[k3], hi = hi2[k3];
   for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
@@ -1157,7 +1174,9 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   V *tw = tile + (size_t)cc * cs;
   int4 *rec = reinterpret_cast<int4 *>(tw + n3);
   const T *tflat = reinterpret_cast<const T *>(tile);
-  const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx, cf = blockIdx.y;
+  // grid (tiles, fields), as the spreader's
+  const int cf = blockIdx.y, tb = blockIdx.x;
+  const int tx = tb % g.ntx, ty = tb / g.ntx;
   const int x0 = tx * g.txr, y0 = ty * g.ty;
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1411,7 +1430,7 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
     StageTimer t(p, 2);
     FixedSizes<T>::pick(g.n[1], [&](auto n) {
       hipLaunchKernelGGL((k_fft_y<T, decltype(n)::value>),
-                         dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, g.F), dim3(NT), p.lds2, st,
+                         dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * g.F), dim3(NT), p.lds2, st,
                          g, p.ax[1], p.tx2, (const V *)p.d_W1, (const V *)p.d_tw[1], p.d_lo2,
                          p.d_hi2, p.d_loff, (V *)p.d_W2);
     });
@@ -1472,7 +1491,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
     StageTimer t(p, 6);
     FixedSizes<T>::pick(g.n[1], [&](auto n) {
       hipLaunchKernelGGL((k_adj_y<T, decltype(n)::value>),
-                         dim3((g.n[0] + p.tx2 - 1) / p.tx2, g.M[2] + 1, Fa), dim3(NT), p.lds2, st,
+                         dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * Fa), dim3(NT), p.lds2, st,
                          g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1], p.d_lo2,
                          p.d_hi2, p.d_loff, (V *)p.d_W1);
     });
