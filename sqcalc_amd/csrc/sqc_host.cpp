@@ -667,6 +667,10 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if ((rc = alloc(&p.d_sidx, sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = upload(&p.d_species, p.species_of.data(), sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = alloc_bins(p))) return rc;
+  // the passes index W1 and W2 with 32-bit element offsets
+  if ((double)g.F * (M3 + 1) * n2 * n1 >= 2147483647.0 ||
+      (double)g.F * std::max(g.nlines, 1) * n1 >= 2147483647.0)
+    return fail(SQC_HIP_ERR_MAXNALLOC, "transform buffers exceed 2^31 elements");
   const size_t w1 = (size_t)g.F * (M3 + 1) * n2 * n1 * vs;
   const size_t w2 = (size_t)g.F * std::max(g.nlines, 1) * n1 * vs;
   if ((rc = alloc((char **)&p.d_W1, w1, p.bytes))) return rc;
@@ -698,6 +702,8 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   }
   if (int e = sqc::kernel_lds_limits(p))
     return hip_fail((hipError_t)e, "setting the LDS size of the kernels");
+  sqc::persistent_grids(p);
+  if ((rc = alloc_wgpart(p))) return rc;
   HIP_TRY(hipDeviceSynchronize(), "plan set-up");
   return ret;
 }
@@ -759,6 +765,8 @@ int apply_tiles(sqc::Plan &p, int txr, int ty, int tx2, int lb) {
     p.d_slots = nullptr;
   }
   if (int e = sqc::kernel_lds_limits(p)) return hip_fail((hipError_t)e, "setting LDS sizes");
+  sqc::persistent_grids(p);
+  if (int rc = alloc_wgpart(p)) return rc;
   drop_graph(p);
   return 0;
 }

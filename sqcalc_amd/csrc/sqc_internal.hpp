@@ -153,6 +153,7 @@ int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const do
                    const double *gP, double *grad);
 int launch_finalize(Plan &p, double *frame_out, int accumulate);
 int kernel_lds_limits(Plan &p);
+void persistent_grids(Plan &p);
 int collect_stage_times(Plan &p);
 
 }  // namespace sqc

@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <type_traits>
 
 #include "sqc_internal.hpp"
@@ -325,13 +326,13 @@ __device__ __forceinline__ int overlap(int i0, int ns, int n, int t0, int tw, in
 // storing any keeps the loads of a thread independent of each other.
 template <int U, typename V, typename Load, typename Store>
 __device__ __forceinline__ void batched_copy(int total, Load load, Store store) {
+  // the loads of a batch are unconditional (the index is clamped to a valid
+  // element) so that they issue back to back; only the stores are guarded
+  if (total <= 0) return;
   for (int base = 0; base < total; base += U * NT) {
     V tmp[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int idx = base + u * NT + (int)threadIdx.x;
-      if (idx < total) tmp[u] = load(idx);
-    }
+    for (int u = 0; u < U; ++u) tmp[u] = load(min(base + u * NT + (int)threadIdx.x, total - 1));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int idx = base + u * NT + (int)threadIdx.x;
@@ -803,8 +804,8 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
   batched_copy<8, V>(
       n2 * tx2,
       [&](int idx) {
-        const int x = x0 + (idx & tm), y = idx >> lt;
-        return x < n1 ? src[(size_t)y * n1 + x] : V{T(0), T(0)};
+        const int x = min(x0 + (idx & tm), n1 - 1), y = idx >> lt;
+        return src[(unsigned)(y * n1 + x)];
       },
       [&](int idx, V v) { lines[(idx & tm) * ls + (idx >> lt)] = v; });
   __syncthreads();
@@ -1546,6 +1547,32 @@ static int lds_limits_t(Plan &p) {
     set((const void *)k_adj_x<T, decltype(n)::value>, p.lds3);
   });
   return err;
+}
+
+// Workgroups of the persistent x pass: what the CUs hold at once by registers
+// and LDS together (the occupancy calculator of the instance that runs),
+// never more than there is work for.
+template <typename T>
+static void persistent_grids_t(Plan &p) {
+  const Geo &g = p.geo;
+  int nx = 0;
+  FixedSizes<T>::pick(g.n[0], [&](auto n) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nx, k_fft_x_bin<T, decltype(n)::value>, NT,
+                                                     p.lds3) != hipSuccess)
+      nx = 0;
+  });
+  (void)hipGetLastError();
+  if (nx > 0) {
+    const int ngroups = (g.nlines + p.lb - 1) / p.lb;
+    p.nblk3 = std::max(1, std::min(ngroups, nx * p.ncu));
+  }
+}
+
+void persistent_grids(Plan &p) {
+  if (p.prec == 1)
+    persistent_grids_t<float>(p);
+  else
+    persistent_grids_t<double>(p);
 }
 
 int kernel_lds_limits(Plan &p) {
