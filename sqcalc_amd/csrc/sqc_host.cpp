@@ -394,7 +394,7 @@ void free_plan(sqc_hip_plan *h) {
   void *bufs[] = {p.d_pos, p.d_X, p.d_i0, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
                   p.d_species, p.d_W1, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
                   p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
-                  p.d_line_off, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
+                  p.d_line_off, p.d_line_d23, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
                   p.d_gnum, p.d_slots, p.d_gs, p.d_rho, h->g_pos, h->g_out, h->g_gs,
                   h->g_grad, h->g_rho};
   for (void *b : bufs)
@@ -677,9 +677,20 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if ((rc = alloc((char **)&p.d_W2, w2, p.bytes))) return rc;
   for (int i = 0; i < 3; ++i)
     if ((rc = upload_twiddles(p, i))) return rc;
+  std::vector<double> phv[3];
   for (int i = 0; i < 3; ++i) {
-    std::vector<double> ph = phihat(g.n[i], g.M[i], ns, g.beta);
-    if ((rc = upload(&p.d_ph[i], ph.data(), ph.size() * sizeof(double), p.bytes))) return rc;
+    phv[i] = phihat(g.n[i], g.M[i], ns, g.beta);
+    if ((rc = upload(&p.d_ph[i], phv[i].data(), phv[i].size() * sizeof(double), p.bytes)))
+      return rc;
+  }
+  {
+    // deconvolution factor of every x line, 1 / (phihat_2(|h2|) phihat_3(h3))
+    std::vector<double> d23(std::max(g.nlines, 1));
+    for (int l = 0; l < g.nlines; ++l) {
+      const int h2 = p.line_info[4 * l], h3 = p.line_info[4 * l + 1];
+      d23[l] = 1.0 / (phv[1][std::abs(h2)] * phv[2][h3]);
+    }
+    if ((rc = upload(&p.d_line_d23, d23.data(), d23.size() * sizeof(double), p.bytes))) return rc;
   }
   if ((rc = upload(&p.d_lo2, p.lo2.data(), p.lo2.size() * 4, p.bytes))) return rc;
   if ((rc = upload(&p.d_hi2, p.hi2.data(), p.hi2.size() * 4, p.bytes))) return rc;

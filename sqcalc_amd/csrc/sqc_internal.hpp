@@ -127,6 +127,7 @@ struct Plan {
   int32_t *d_lo2 = nullptr, *d_hi2 = nullptr, *d_loff = nullptr, *d_line_info = nullptr,
           *d_line_off = nullptr;
   uint16_t *d_stab = nullptr;
+  double *d_line_d23 = nullptr;  // per x line: 1 / (phihat_2 phihat_3)
   double *d_den = nullptr, *d_cnt = nullptr;
   double *d_wgpart = nullptr, *d_acc = nullptr, *d_frame = nullptr;
   double *d_gnum = nullptr;

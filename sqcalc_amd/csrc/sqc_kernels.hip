@@ -846,26 +846,52 @@ __device__ __forceinline__ double qlen_of(const Geo &g, int h1, int h2, int h3) 
 }
 
 // Per-group line tables staged in LDS by the x-pass kernels.
+// Shell indices a thread of the x pass prefetches per line group (groups
+// hold up to kPreShell * NT candidates before the rest is read directly).
+constexpr int kPreShell = 8;
+
 struct LineGroup {
   int off[kMaxLb + 1];     // candidate offset of each line (and the end)
   int h[kMaxLb][3];        // h2, h3, lo1
   double d23[kMaxLb];      // 1 / (phihat_2(|h2|) phihat_3(h3))
 };
 
+// The line table of a group, read by the first nl + 1 threads: issue() starts
+// the loads, put() stores them in LDS once they are needed, so the loads can
+// be in flight together with the rows'.
+struct GroupLoad {
+  int4 info;
+  int off;
+  double d23;
+  __device__ __forceinline__ void issue(int l0, int nl, const int4 *__restrict__ line_info,
+                                        const int32_t *__restrict__ line_off,
+                                        const double *__restrict__ line_d23) {
+    const int tid = threadIdx.x;
+    if (tid <= nl) off = line_off[l0 + tid];
+    if (tid < nl) {
+      info = line_info[l0 + tid];
+      d23 = line_d23[l0 + tid];
+    }
+  }
+  __device__ __forceinline__ void put(LineGroup &lg, int nl) const {
+    const int tid = threadIdx.x;
+    if (tid <= nl) lg.off[tid] = off;
+    if (tid < nl) {
+      lg.h[tid][0] = info.x;
+      lg.h[tid][1] = info.y;
+      lg.h[tid][2] = info.z;
+      lg.d23[tid] = d23;
+    }
+  }
+};
+
 __device__ __forceinline__ void load_group(LineGroup &lg, int l0, int nl,
                                            const int4 *__restrict__ line_info,
                                            const int32_t *__restrict__ line_off,
-                                           const double *__restrict__ ph2,
-                                           const double *__restrict__ ph3) {
-  const int tid = threadIdx.x;
-  if (tid <= nl) lg.off[tid] = line_off[l0 + tid];
-  if (tid < nl) {
-    const int4 info = line_info[l0 + tid];
-    lg.h[tid][0] = info.x;
-    lg.h[tid][1] = info.y;
-    lg.h[tid][2] = info.z;
-    lg.d23[tid] = 1.0 / (ph2[abs(info.x)] * ph3[info.y]);
-  }
+                                           const double *__restrict__ line_d23) {
+  GroupLoad gl;
+  gl.issue(l0, nl, line_info, line_off, line_d23);
+  gl.put(lg, nl);
 }
 
 template <typename T, int N>
@@ -876,8 +902,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
                                                   const int32_t *__restrict__ line_off,
                                                   const uint16_t *__restrict__ stab,
                                                   const double *__restrict__ ph1,
-                                                  const double *__restrict__ ph2,
-                                                  const double *__restrict__ ph3,
+                                                  const double *__restrict__ line_d23,
                                                   double *__restrict__ wgpart,
                                                   typename C2<T>::t *__restrict__ rho_out,
                                                   int64_t ncand, double *__restrict__ gnum) {
@@ -899,26 +924,33 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
     const int l0 = grp * lb;
     const int nl = min(lb, g.nlines - l0);
     __syncthreads();
-    load_group(lg, l0, nl, line_info, line_off, ph2, ph3);
+    GroupLoad gl;
+    gl.issue(l0, nl, line_info, line_off, line_d23);
     const float inv_n1 = 1.0f / (float)n1, inv_f = 1.0f / (float)F;
     batched_copy<8, V>(
         nl * F * n1,
         [&](int idx) {
           const int r = small_div(idx, inv_n1), x = idx - r * n1;
           const int li = small_div(r, inv_f), f = r - li * F;
-          return W2[((size_t)f * g.nlines + l0 + li) * n1 + x];
+          return W2[(unsigned)((f * g.nlines + l0 + li) * n1 + x)];
         },
         [&](int idx, V v) {
           const int r = small_div(idx, inv_n1);
           rows[r * rs + (idx - r * n1)] = v;
         });
-    __syncthreads();
-    for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
+    gl.put(lg, nl);
     __syncthreads();
     const int c0 = lg.off[0], c1 = lg.off[nl];
-    for (int c = c0 + tid; c < c1; c += NT) {
-      const unsigned s = stab[c];
-      if (s == kNoShell) continue;
+    // shell indices of this thread's candidates, in flight during the DFT
+    uint16_t sv[kPreShell];
+    if (c1 > c0) {
+#pragma unroll
+      for (int k = 0; k < kPreShell; ++k) sv[k] = stab[min(c0 + tid + k * NT, c1 - 1)];
+    }
+    for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
+    __syncthreads();
+    auto bin_mode = [&](int c, unsigned s) {
+      if (s == kNoShell) return;
       int li = 0;
       while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
@@ -955,6 +987,15 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
           for (int b = a; b < S; ++b, ++pidx)
             atomicAdd(&acc[pidx * nq + s], 2.0 * (rr[a] * rr[b] + ri[a] * ri[b]));
       }
+    };
+    // one copy of the body: the prefetched index of round k is picked by a
+    // compare-select chain, so sv stays in registers
+#pragma unroll 1
+    for (int k = 0, c = c0 + tid; c < c1; ++k, c += NT) {
+      unsigned sh = sv[0];
+#pragma unroll
+      for (int u = 1; u < kPreShell; ++u) sh = (k == u) ? sv[u] : sh;
+      bin_mode(c, k < kPreShell ? sh : (unsigned)stab[c]);
     }
   }
   __syncthreads();
@@ -1026,8 +1067,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
                                               const int32_t *__restrict__ line_off,
                                               const uint16_t *__restrict__ stab,
                                               const double *__restrict__ ph1,
-                                              const double *__restrict__ ph2,
-                                              const double *__restrict__ ph3,
+                                              const double *__restrict__ line_d23,
                                               const double *__restrict__ den,
                                               const double *__restrict__ cnt,
                                               const double *__restrict__ gS,
@@ -1048,7 +1088,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
     const int l0 = grp * lb;
     const int nl = min(lb, g.nlines - l0);
     __syncthreads();
-    load_group(lg, l0, nl, line_info, line_off, ph2, ph3);
+    load_group(lg, l0, nl, line_info, line_off, line_d23);
     for (int idx = tid; idx < nl * Fa * rs; idx += NT) rows[idx] = V{T(0), T(0)};
     __syncthreads();
     const int c0 = lg.off[0], c1 = lg.off[nl];
@@ -1443,7 +1483,7 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
       hipLaunchKernelGGL((k_fft_x_bin<T, decltype(n)::value>), dim3(p.nblk3), dim3(NT), p.lds3,
                          st, g, p.ax[0], p.lb, ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
                          (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0],
-                         p.d_ph[1], p.d_ph[2], p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
+                         p.d_line_d23, p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
                          mode == 0 ? p.d_gnum : nullptr);
     });
   }
@@ -1484,7 +1524,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
       hipLaunchKernelGGL((k_adj_x<T, decltype(n)::value>), dim3(p.nblk3), dim3(NT), p.lds3, st, g,
                          p.ax[0], p.lb, ngroups, Fa, (const V *)rho, (int64_t)p.ncand,
                          (const V *)p.d_tw[0], (const int4 *)p.d_line_info, p.d_line_off,
-                         p.d_stab, p.d_ph[0], p.d_ph[1], p.d_ph[2], p.d_den, p.d_cnt, gS, gP,
+                         p.d_stab, p.d_ph[0], p.d_line_d23, p.d_den, p.d_cnt, gS, gP,
                          (V *)p.d_W2);
     });
   }
