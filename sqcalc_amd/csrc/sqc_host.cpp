@@ -273,7 +273,9 @@ size_t vsize(const sqc::Plan &p) { return p.prec == 1 ? 8 : 16; }
 
 size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
   const sqc::Geo &g = p.geo;
-  return ((size_t)(txr / 2) * ty * g.cs + g.n[2]) * vsize(p) + sqc::kStageCap * (16 + 2 * 4);
+  // the tile and the staged atom records; the DFT twiddles are read from
+  // global memory (L1-resident), which leaves room for more workgroups
+  return (size_t)(txr / 2) * ty * g.cs * vsize(p) + sqc::kStageCap * (16 + 2 * 4);
 }
 
 // y-line stride: the transposed copies put lane (line, y) at line * ls2 + y
@@ -287,7 +289,7 @@ int ystride(const sqc::Plan &p, int tx2) {
 }
 
 size_t ypass_lds(const sqc::Plan &p, int tx2) {
-  return ((size_t)tx2 * ystride(p, tx2) + p.geo.n[1]) * vsize(p);
+  return (size_t)tx2 * ystride(p, tx2) * vsize(p);
 }
 
 int ilog2i(int v) {

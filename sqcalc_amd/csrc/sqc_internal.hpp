@@ -43,7 +43,7 @@ constexpr int kMaxLine = 1024;
 // Most lines an x-pass workgroup transforms per step.
 constexpr int kMaxLb = 8;
 // Atoms staged into LDS per round of the spreader / gather.
-constexpr int kStageCap = 512;
+constexpr int kStageCap = 256;
 
 // Everything a kernel needs to know about the geometry, passed by value.
 struct Geo {

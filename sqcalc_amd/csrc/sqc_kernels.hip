@@ -684,8 +684,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], cs = g.cs;
   const int ccx = g.txr / 2, cc = ccx * g.ty;
   V *tile = reinterpret_cast<V *>(smem);
-  V *tw = tile + (size_t)cc * cs;
-  int4 *rec = reinterpret_cast<int4 *>(tw + n3);
+  int4 *rec = reinterpret_cast<int4 *>(tile + (size_t)cc * cs);
   unsigned short *slist = reinterpret_cast<unsigned short *>(rec + kStageCap);
   T *tflat = reinterpret_cast<T *>(tile);
   // grid (tiles, fields) in hardware order: measured faster here than the
@@ -696,7 +695,6 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
-  for (int i = tid; i < n3; i += NT) tw[i] = twz[i];
   if (tid < NW) st.slab_n[tid] = 0;
   stage_window(st, g, tx, ty, f, start);
 
@@ -752,7 +750,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     __syncthreads();
   }
   __syncthreads();
-  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, tw, lane);
+  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, twz, lane);
   __syncthreads();
   // split the packed pairs and write h3 = 0..M3.  txr * ty is a power of two
   // that divides NT, so every thread keeps one (xl, yl) column and steps h3.
@@ -791,14 +789,12 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
   extern __shared__ __align__(16) unsigned char smem[];
   const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
   V *lines = reinterpret_cast<V *>(smem);
-  V *tw = lines + (size_t)tx2 * ls;
   const int nbx = (g.n[0] + tx2 - 1) / tx2;
   const int blk = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
   const int x0 = (blk % nbx) * tx2, k3 = (blk / nbx) % (g.M[2] + 1), f = blk / (nbx * (g.M[2] + 1));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lo = lo2[k3], hi = hi2[k3];
   if (hi < lo) return;
-  for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
   const V *src = W1 + ((size_t)f * (M3 + 1) + k3) * n2 * n1;
   const int lt = g.ltx2, tm = tx2 - 1;
   batched_copy<8, V>(
@@ -809,7 +805,7 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
       },
       [&](int idx, V v) { lines[(idx & tm) * ls + (idx >> lt)] = v; });
   __syncthreads();
-  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, tw, lane);
+  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, twy, lane);
   __syncthreads();
   const int nl = hi - lo + 1;
   V *dst = W2 + ((size_t)f * g.nlines + loff[k3]) * n1;
@@ -1158,13 +1154,11 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
   extern __shared__ __align__(16) unsigned char smem[];
   const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
   V *lines = reinterpret_cast<V *>(smem);
-  V *tw = lines + (size_t)tx2 * ls;
   const int nbx = (g.n[0] + tx2 - 1) / tx2;
   const int blk = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
   const int x0 = (blk % nbx) * tx2, k3 = (blk / nbx) % (g.M[2] + 1), cf = blk / (nbx * (g.M[2] + 1));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lo = lo2[k3], hi = hi2[k3];
-  for (int i = tid; i < n2; i += NT) tw[i] = twy[i];
   for (int i = tid; i < tx2 * ls; i += NT) lines[i] = V{T(0), T(0)};
   __syncthreads();
   const V *src = W2 + ((size_t)cf * g.nlines + (hi >= lo ? loff[k3] : 0)) * n1;
@@ -1181,7 +1175,7 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
         if (x0 + (idx & tm) < n1) lines[(idx & tm) * ls + yi] = v;
       });
   __syncthreads();
-  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, tw, lane);
+  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, twy, lane);
   __syncthreads();
   V *dst = W1 + ((size_t)cf * (M3 + 1) + k3) * n2 * n1;
   for (int idx = tid; idx < n2 * tx2; idx += NT) {
@@ -1212,8 +1206,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2], cs = g.cs;
   const int ccx = g.txr / 2, cc = ccx * g.ty;
   V *tile = reinterpret_cast<V *>(smem);
-  V *tw = tile + (size_t)cc * cs;
-  int4 *rec = reinterpret_cast<int4 *>(tw + n3);
+  int4 *rec = reinterpret_cast<int4 *>(tile + (size_t)cc * cs);
   const T *tflat = reinterpret_cast<const T *>(tile);
   // grid (tiles, fields), as the spreader's
   const int cf = blockIdx.y, tb = blockIdx.x;
@@ -1222,7 +1215,6 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
-  for (int i = tid; i < n3; i += NT) tw[i] = twz[i];
   __syncthreads();
   const int pw = txw / 2;
   const V *src = W1 + (size_t)cf * (M3 + 1) * n2 * n1;
@@ -1245,7 +1237,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         if (k3 > 0) tile[(size_t)col * cs + (n3 - k3)] = V{A.x + B.y, B.x - A.y};
       });
   __syncthreads();
-  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, tw, lane);
+  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, twz, lane);
   stage_window(st, g, tx, ty, cf, start);
 
   const int ns = g.ns;
