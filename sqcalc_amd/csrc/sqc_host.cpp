@@ -744,25 +744,36 @@ struct OrderScope {
 
 // Forward time of one frame (ms, best of two after a warm-up), nothing
 // accumulated.
-int time_forward(sqc::Plan &p, const double *pos_dev, float *best) {
-  hipEvent_t a = nullptr, b = nullptr;
-  HIP_TRY(hipEventCreate(&a), "hipEventCreate");
-  if (hipEventCreate(&b) != hipSuccess) {
-    (void)hipEventDestroy(a);
-    return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
+// Per-stage forward times of one frame (ms, best of three after a warm-up),
+// nothing accumulated: stage[0] atom sort, [1] spread + z, [2] y, [3] x + bin.
+int time_stages(sqc::Plan &p, const double *pos_dev, double stage[4]) {
+  for (int i = 0; i < 4; ++i) stage[i] = 1e30;
+  const int saved = p.profiling;
+  (void)sqc::collect_stage_times(p);
+  double keep_ms[sqc::kNumStages];
+  int64_t keep_n[sqc::kNumStages];
+  for (int i = 0; i < sqc::kNumStages; ++i) {
+    keep_ms[i] = p.stage_ms[i];
+    keep_n[i] = p.stage_n[i];
   }
-  *best = 1e30f;
+  p.profiling = 1;
   int rc = 0;
-  for (int it = 0; it < 3 && !rc; ++it) {
-    (void)hipEventRecord(a, p.stream);
+  for (int it = 0; it < 4 && !rc; ++it) {
+    for (int i = 0; i < sqc::kNumStages; ++i) {
+      p.stage_ms[i] = 0.0;
+      p.stage_n[i] = 0;
+    }
     rc = sqc::launch_forward(p, pos_dev, 2, nullptr, nullptr);
-    (void)hipEventRecord(b, p.stream);
-    if (!rc) rc = (int)hipEventSynchronize(b);
-    float ms = 0.f;
-    if (!rc && it > 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess) *best = std::min(*best, ms);
+    if (!rc) rc = sqc::collect_stage_times(p);
+    if (!rc && it > 0)
+      for (int i = 0; i < 4; ++i) stage[i] = std::min(stage[i], p.stage_ms[i]);
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  // the caller's own stage statistics are left as they were
+  for (int i = 0; i < sqc::kNumStages; ++i) {
+    p.stage_ms[i] = keep_ms[i];
+    p.stage_n[i] = keep_n[i];
+  }
+  p.profiling = saved;
   return rc ? hip_fail((hipError_t)rc, "tuning launch") : 0;
 }
 
@@ -784,43 +795,52 @@ int apply_tiles(sqc::Plan &p, int txr, int ty, int tx2, int lb) {
   return 0;
 }
 
-// Autotuner (config.tune): on the first frame, time the forward for every
-// spread tile that fits the LDS, then for every y-pass column chunk and every
-// x-pass line-group size with the best choices so far, and keep the fastest.
-// Nothing is accumulated while tuning.
+// Autotuner (config.tune): on the first frame, time every spread tile that
+// fits the LDS (atom sort + spread stages), then every y-pass column chunk (y
+// stage) and every x-pass line-group size (x stage) with the best choices so
+// far, and keep the fastest.  Nothing is accumulated while tuning.
 int tune(sqc::Plan &p, const double *pos_dev) {
   p.tuned = 1;
   if (!p.cfg.tune) return 0;
+  // each choice is judged on the stages it changes, and a candidate replaces
+  // the current choice only when it is more than 1% faster
+  constexpr double kGain = 0.99;
   int btxr = p.geo.txr, bty = p.geo.ty, btx2 = p.tx2, blb = p.lb;
-  float best = 1e30f;
+  double st[4];
+  if (int rc = time_stages(p, pos_dev, st)) return rc;
+  double best = st[0] + st[1];
   for (int c = 0; c < kNumTiles; ++c) {
-    if (!tile_fits(p, kTiles[c][0], kTiles[c][1], kLdsCap)) continue;
-    if (int rc = apply_tiles(p, kTiles[c][0], kTiles[c][1], btx2, blb)) return rc;
-    float ms;
-    if (int rc = time_forward(p, pos_dev, &ms)) return rc;
-    if (ms < best) {
-      best = ms;
-      btxr = kTiles[c][0];
-      bty = kTiles[c][1];
+    const int txr = kTiles[c][0], ty = kTiles[c][1];
+    if ((txr == btxr && ty == bty) || !tile_fits(p, txr, ty, kLdsCap)) continue;
+    if (int rc = apply_tiles(p, txr, ty, btx2, blb)) return rc;
+    if (int rc = time_stages(p, pos_dev, st)) return rc;
+    if (st[0] + st[1] < kGain * best) {
+      best = st[0] + st[1];
+      btxr = txr;
+      bty = ty;
     }
   }
+  if (int rc = apply_tiles(p, btxr, bty, btx2, blb)) return rc;
+  if (int rc = time_stages(p, pos_dev, st)) return rc;
+  best = st[2];
   for (int tx2 = 16; tx2 >= 2; tx2 /= 2) {
     if (ypass_lds(p, tx2) > kLdsCap || tx2 == btx2) continue;
     if (int rc = apply_tiles(p, btxr, bty, tx2, blb)) return rc;
-    float ms;
-    if (int rc = time_forward(p, pos_dev, &ms)) return rc;
-    if (ms < best) {
-      best = ms;
+    if (int rc = time_stages(p, pos_dev, st)) return rc;
+    if (st[2] < kGain * best) {
+      best = st[2];
       btx2 = tx2;
     }
   }
+  if (int rc = apply_tiles(p, btxr, bty, btx2, blb)) return rc;
+  if (int rc = time_stages(p, pos_dev, st)) return rc;
+  best = st[3];
   for (int lb = sqc::kMaxLb; lb >= 1; lb /= 2) {
     if (xpass_lds(p, lb) > kLdsCap || lb == blb) continue;
     if (int rc = apply_tiles(p, btxr, bty, btx2, lb)) return rc;
-    float ms;
-    if (int rc = time_forward(p, pos_dev, &ms)) return rc;
-    if (ms < best) {
-      best = ms;
+    if (int rc = time_stages(p, pos_dev, st)) return rc;
+    if (st[3] < kGain * best) {
+      best = st[3];
       blb = lb;
     }
   }
