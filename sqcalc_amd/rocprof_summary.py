@@ -1,9 +1,10 @@
 """Short text summaries of rocprofv3 CSV output for the S(q) kernels.
 
-  python -m sqcalc_amd.rocprof_summary stats STATS_CSV --command "..." > out.md
+  python -m sqcalc_amd.rocprof_summary stats STATS_CSV [--trace TRACE_CSV] --command "..."
       per-kernel calls / total / average duration from a
       ``rocprofv3 --kernel-trace --stats --output-format csv`` run
-      (the ``*_kernel_stats.csv`` file).
+      (the ``*_kernel_stats.csv`` file), and with --trace the same
+      dispatches grouped by launch shape.
 
   python -m sqcalc_amd.rocprof_summary traffic --fetch FETCH_CSV --write WRITE_CSV
          --copy-bytes B --json out.json --command "..." > out.md
@@ -57,14 +58,33 @@ def read_stats(path):
 
 
 def read_counter(path, counter):
-    """Mean counter value per dispatch, per kernel name."""
-    acc = collections.defaultdict(list)
+    """Mean counter value per dispatch, per kernel name, over the kernel's most
+    frequent launch shape (grid and LDS size): the tuner's trial launches of
+    other tiles are left out."""
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
     with open(path) as fh:
         for r in csv.DictReader(fh):
             if r["Counter_Name"] != counter:
                 continue
-            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+            shape = (r.get("Grid_Size"), r.get("LDS_Block_Size"))
+            acc[r["Kernel_Name"]][shape].append(float(r["Counter_Value"]))
+    out = {}
+    for k, shapes in acc.items():
+        v = max(shapes.values(), key=len)
+        out[k] = sum(v) / len(v)
+    return out
+
+
+def read_trace(path):
+    """Dispatches of a --kernel-trace CSV grouped by (kernel, grid, LDS): the
+    tuner's trial launches then fall in groups of their own."""
+    groups = collections.OrderedDict()
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            grid = (int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]), int(r["Grid_Size_Z"]))
+            key = (r["Kernel_Name"], grid, int(r["LDS_Block_Size"]))
+            groups.setdefault(key, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    return groups
 
 
 def cmd_stats(args):
@@ -76,6 +96,18 @@ def cmd_stats(args):
     for name, calls, tot, avg, pct in rows[:args.top]:
         print("| `%s` | %d | %.3f | %.2f | %.2f |" % (short_name(name), calls, tot / 1e6,
                                                      avg / 1e3, pct))
+    if args.trace:
+        groups = read_trace(args.trace)
+        print("\nBy launch shape (the same kernels, dispatches grouped by grid and LDS "
+              "size; the tuner's trial tiles form groups of their own):\n")
+        print("| kernel | grid (threads) | LDS bytes | calls | average us |")
+        print("|---|---|---:|---:|---:|")
+        keep = [k for k in groups if base_name(k[0]) in STAGE_OF]
+        keep.sort(key=lambda k: -sum(groups[k]))
+        for k in keep[:args.top]:
+            d = groups[k]
+            print("| `%s` | %s | %d | %d | %.2f |" % (short_name(k[0]), "x".join(map(str, k[1])),
+                                                      k[2], len(d), sum(d) / len(d) / 1e3))
 
 
 def cmd_traffic(args):
@@ -124,6 +156,7 @@ def main(argv=None):
     a.add_argument("csv")
     a.add_argument("--command", default="")
     a.add_argument("--top", type=int, default=20)
+    a.add_argument("--trace", help="the *_kernel_trace.csv of the same run")
     b = sub.add_parser("traffic")
     b.add_argument("--fetch", required=True)
     b.add_argument("--write", required=True)
