@@ -318,6 +318,12 @@ void set_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
   g.txr = txr;
   g.ty = ty;
   p.lds1 = spread_lds(p, txr, ty);
+  {
+    // the gather adds a per-wave scratch of the kernel values of the atoms
+    // of one round: (64 / lanes per atom) atoms x 6 ns values
+    const int lpa = std::max(8, txr * ty);
+    p.lds1g = p.lds1 + (size_t)(sqc::kThreads / 64) * (64 / lpa) * 6 * g.ns * (vsize(p) / 2);
+  }
   g.ltxr = ilog2i(g.txr);
   g.lty = ilog2i(g.ty);
   g.ntx = (n1 + g.txr - 1) / g.txr;

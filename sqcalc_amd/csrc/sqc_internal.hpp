@@ -44,6 +44,7 @@ constexpr int kMaxLine = 1024;
 constexpr int kMaxLb = 8;
 // Atoms staged into LDS per round of the spreader / gather.
 constexpr int kStageCap = 256;
+constexpr int kThreads = 256;   // threads per workgroup of the LDS kernels
 
 // Everything a kernel needs to know about the geometry, passed by value.
 struct Geo {
@@ -106,6 +107,7 @@ struct Plan {
   int wgpart_rows = 0;        // rows d_wgpart was sized for
   int ncu = 1;                // compute units of the device
   size_t lds1 = 0, lds2 = 0, lds3 = 0;
+  size_t lds1g = 0;           // gather: lds1 plus its per-wave kernel-value scratch
   int64_t nframes = 0;
   int64_t batch_cap = 1;
   int nbt_alloc = 0;           // bins the sort counters were sized for

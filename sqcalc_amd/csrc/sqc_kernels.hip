@@ -32,7 +32,7 @@
 
 namespace sqc {
 
-constexpr int NT = 256;  // threads per workgroup for every LDS kernel
+constexpr int NT = kThreads;  // threads per workgroup for every LDS kernel
 constexpr int NW = NT / 64;
 
 template <typename T> struct C2;
@@ -1242,51 +1242,65 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
 
   const int ns = g.ns;
   const int S2 = g.sx * g.sy;
+  // Gather: the tile is only read, so one wave serves several atoms at once:
+  // lpa lanes per atom (one per xy column of its stencil in the tile, at
+  // least 8), ga = 64 / lpa atoms per round.  The kernel values of a round's
+  // atoms go to a per-wave scratch; each lane sums its column over z in the
+  // path precision, and the three derivative sums are reduced in FP64 over
+  // the atom's lanes.
+  const int lpa = max(8, g.txr * g.ty), ga = 64 / lpa;
+  const int grp = lane / lpa, l = lane - grp * lpa;
+  T *scr = reinterpret_cast<T *>(rec + kStageCap) + (size_t)wave * ga * 6 * ns;
   for (int base = 0; base < st.total; base += NT) {
     const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a);
     const bool last = base + NT >= st.total;
     if (!last && nst <= kStageCap - NT) continue;
-    for (int s = wave; s < nst; s += NW) {
-      const int4 r = rec[s];
-      const int j = r.x;
+    for (int s0 = wave * ga; s0 < nst; s0 += NW * ga) {
+      const int na = min(ga, nst - s0);
+      // phi then dphi of the round's atoms: [atom][6 ns]
+      for (int k = lane; k < na * 6 * ns; k += 64) {
+        const int aa = k / (6 * ns), kk = k - aa * 6 * ns;
+        const int j = rec[s0 + aa].x;
+        scr[k] = kk < 3 * ns ? phi[(size_t)j * 3 * ns + kk] : dphi[(size_t)j * 3 * ns + kk - 3 * ns];
+      }
+      wave_sync();
+      const bool act = grp < na;
+      const int4 r = rec[s0 + min(grp, na - 1)];
       const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
       const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
       const int a = r.w;
-      const T ph = lane < 3 * ns ? phi[(size_t)j * 3 * ns + lane] : T(0);
-      const T dph = lane < 3 * ns ? dphi[(size_t)j * 3 * ns + lane] : T(0);
+      const bool cell = act && l < ox * oy;
+      const int lc = cell ? l : 0;
+      const int iy = small_div(lc, __builtin_amdgcn_rcpf((float)ox));
+      const int ix = lc - iy * ox;
+      const T *ps = scr + grp * 6 * ns;
       double gx = 0.0, gy = 0.0, gz = 0.0;
-      const int cells = ox * oy * ns;
-      const float inv_ns = 1.0f / (float)ns, inv_ox = 1.0f / (float)ox;
-      for (int c0 = 0; c0 < cells; c0 += 64) {
-        const int c = c0 + lane;
-        const int cv = c < cells ? c : 0;
-        const int t1 = small_div(cv, inv_ns);
-        const int kz = cv - t1 * ns;
-        const int iy = small_div(t1, inv_ox);
-        const int ix = t1 - iy * ox;
-        const T fx = __shfl(ph, kxl + ix), dfx = __shfl(dph, kxl + ix);
-        const T fy = __shfl(ph, ns + kyl + iy), dfy = __shfl(dph, ns + kyl + iy);
-        const T fz = __shfl(ph, 2 * ns + kz), dfz = __shfl(dph, 2 * ns + kz);
-        if (c < cells) {
-          const int xl = xll + ix, yl = yll + iy;
-          int z = a + kz;
-          if (z >= n3) z -= n3;
-          const int col = yl * ccx + (xl >> 1);
-          const double v = (double)tflat[2 * ((size_t)col * cs + (z)) + (xl & 1)];
-          gx += v * (double)(dfx * fy * fz);
-          gy += v * (double)(fx * dfy * fz);
-          gz += v * (double)(fx * fy * dfz);
+      if (cell) {
+        const T fx = ps[kxl + ix], fy = ps[ns + kyl + iy];
+        const T dfx = ps[3 * ns + kxl + ix], dfy = ps[4 * ns + kyl + iy];
+        const int xl = xll + ix, yl = yll + iy;
+        const T *colv = tflat + 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
+        T sv = T(0), sd = T(0);
+        int z = a;
+        for (int kz = 0; kz < ns; ++kz) {
+          const T v = colv[2 * z];
+          sv += v * ps[2 * ns + kz];
+          sd += v * ps[5 * ns + kz];
+          z = (z + 1 == n3) ? 0 : z + 1;
         }
+        gx = (double)(dfx * fy) * (double)sv;
+        gy = (double)(fx * dfy) * (double)sv;
+        gz = (double)(fx * fy) * (double)sd;
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
+      for (int off = lpa >> 1; off > 0; off >>= 1) {
         gx += __shfl_xor(gx, off);
         gy += __shfl_xor(gy, off);
         gz += __shfl_xor(gz, off);
       }
-      if (lane == 0) {
+      if (act && l == 0) {
         // slot of this (atom, tile) pair: tiles counted from the one that
         // holds the first stencil point
+        const int j = r.x;
         const int4 i0 = i0a[j];
         const int xs = wrapi(i0.x, n1), ys = wrapi(i0.y, n2);
         const int dx = wrapi(tx - xs / g.txr, g.ntx);
@@ -1296,6 +1310,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         o[1] = gy;
         o[2] = gz;
       }
+      wave_sync();  // the next round overwrites the scratch
     }
     __syncthreads();
     if (tid == 0) st.nstaged = 0;
@@ -1533,7 +1548,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
     StageTimer t(p, 7);
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
       hipLaunchKernelGGL((k_adj_z_gather<T, decltype(n)::value>), dim3(g.ntx * g.nty, Fa),
-                         dim3(NT), p.lds1, st, g, p.ax[2], (const V *)p.d_W1,
+                         dim3(NT), p.lds1g, st, g, p.ax[2], (const V *)p.d_W1,
                          (const V *)p.d_tw[2], p.d_i0, (const T *)p.d_phi, (const T *)p.d_dphi,
                          p.d_sidx, p.d_start, p.d_slots);
     });
@@ -1568,7 +1583,7 @@ static int lds_limits_t(Plan &p) {
   };
   FixedSizes<T>::pick(p.geo.n[2], [&](auto n) {
     set((const void *)k_spread_z<T, decltype(n)::value>, p.lds1);
-    set((const void *)k_adj_z_gather<T, decltype(n)::value>, p.lds1);
+    set((const void *)k_adj_z_gather<T, decltype(n)::value>, p.lds1g);
   });
   FixedSizes<T>::pick(p.geo.n[1], [&](auto n) {
     set((const void *)k_fft_y<T, decltype(n)::value>, p.lds2);
