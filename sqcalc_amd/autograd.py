@@ -24,6 +24,21 @@ def ctypes_void(v):
     return ctypes.c_void_p(v)
 
 
+def _norms(sf, device):
+    """Per-shell factors 1/den (0 where den = 0) and 1/(N count) on the
+    device, built once per plan and device: a host-to-device copy per frame
+    would synchronise the stream."""
+    cache = sf.__dict__.setdefault("_device_norms", {})
+    key = str(device)
+    if key not in cache:
+        import numpy as np
+        den = np.asarray(sf.den, dtype=np.float64)
+        inv_den = np.where(den > 0, 1.0 / np.where(den > 0, den, 1.0), 0.0)
+        inv_pair = 1.0 / (sf.natoms * np.maximum(np.asarray(sf.shell_count, np.float64), 1.0))
+        cache[key] = (torch.tensor(inv_den, device=device), torch.tensor(inv_pair, device=device))
+    return cache[key]
+
+
 class _SqFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pos, sf):
@@ -38,10 +53,9 @@ class _SqFunction(torch.autograd.Function):
         rc = sf.lib.sqc_hip_forward_device(sf.plan, pos.data_ptr(), 1, out.data_ptr(),
                                            rho.data_ptr(), _stream())
         _lib.check(rc, "sqc_hip_forward_device")
-        den = torch.as_tensor(sf.den, device=pos.device)
-        cnt = torch.as_tensor(sf.shell_count, device=pos.device, dtype=torch.float64)
-        S = torch.where(den > 0, out[0] / torch.where(den > 0, den, 1.0), 0.0)
-        P = out[1:] / (sf.natoms * torch.clamp(cnt, min=1.0))
+        inv_den, inv_pair = _norms(sf, pos.device)
+        S = out[0] * inv_den
+        P = out[1:] * inv_pair
         ctx.sf = sf
         ctx.save_for_backward(pos, rho)
         return S, P

@@ -1088,9 +1088,21 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
     for (int idx = tid; idx < nl * Fa * rs; idx += NT) rows[idx] = V{T(0), T(0)};
     __syncthreads();
     const int c0 = lg.off[0], c1 = lg.off[nl];
-    for (int c = c0 + tid; c < c1; c += NT) {
-      const unsigned s = stab[c];
-      if (s == kNoShell) continue;
+    // shell indices and field-0 amplitudes of this thread's first KP
+    // candidates, all in flight at once
+    constexpr int KP = sizeof(T) == 4 ? kPreShell : kPreShell / 2;
+    uint16_t sv[KP];
+    V rv[KP];
+    if (c1 > c0) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int c = min(c0 + tid + k * NT, c1 - 1);
+        sv[k] = stab[c];
+        rv[k] = rho[c];
+      }
+    }
+    auto amplify = [&](int c, unsigned s, V r0) {
+      if (s == kNoShell) return;
       int li = 0;
       while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
@@ -1098,7 +1110,9 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       const double d = lg.d23[li] * inv1[abs(h1)];
       const int ix = h1 < 0 ? h1 + n1 : h1;
       double rr[kMaxSpecies], ri[kMaxSpecies], w[kMaxSpecies];
-      for (int f = 0; f < F; ++f) {
+      rr[0] = r0.x;
+      ri[0] = r0.y;
+      for (int f = 1; f < F; ++f) {
         const V v = rho[(size_t)f * ncand + c];
         rr[f] = v.x;
         ri[f] = v.y;
@@ -1130,14 +1144,30 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
         }
         rows[((size_t)li * Fa + cf) * rs + (ix)] = V{(T)(cr * d), (T)(ci * d)};
       }
+    };
+#pragma unroll 1
+    for (int k = 0, c = c0 + tid; c < c1; ++k, c += NT) {
+      unsigned sh = sv[0];
+      V r0 = rv[0];
+#pragma unroll
+      for (int u = 1; u < KP; ++u) {
+        sh = (k == u) ? sv[u] : sh;
+        r0 = (k == u) ? rv[u] : r0;
+      }
+      if (k >= KP) {
+        sh = stab[c];
+        r0 = rho[c];
+      }
+      amplify(c, sh, r0);
     }
     __syncthreads();
     for (int r = wave; r < nl * Fa; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
     __syncthreads();
+    const float inv_n1 = 1.0f / (float)n1, inv_fa = 1.0f / (float)Fa;
     for (int idx = tid; idx < nl * Fa * n1; idx += NT) {
-      const int x = idx % n1, r = idx / n1;
-      const int li = r / Fa, cf = r % Fa;
-      W2[((size_t)cf * g.nlines + l0 + li) * n1 + x] = rows[r * rs + (x)];
+      const int r = small_div(idx, inv_n1), x = idx - r * n1;
+      const int li = small_div(r, inv_fa), cf = r - li * Fa;
+      W2[(unsigned)((cf * g.nlines + l0 + li) * n1 + x)] = rows[r * rs + x];
     }
   }
 }
