@@ -509,7 +509,8 @@ __device__ __forceinline__ void bin_window(int t, int h, int nt, int *first, int
 // stencil reaches the tile are compacted into LDS in bin order (a block scan
 // of the overlap flags), so every round is deterministic.
 //   rec.x = atom, rec.y = kx | xl << 8 | ox << 16, rec.z = ky | yl << 8 | oy << 16,
-//   rec.w = first z point of the stencil, wrapped into [0, n3)
+//   rec.w = first z point of the stencil, wrapped into [0, n3), | species << 16
+//           (single-field spreads, which weight each atom)
 // ---------------------------------------------------------------------------
 constexpr int kMaxWindowBins = 128;
 
@@ -558,7 +559,8 @@ __device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
 // staged; returns (block-uniform) the number staged so far.
 __device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0, int txw, int y0,
                            int tyw, const int32_t *__restrict__ sidx,
-                           const int4 *__restrict__ i0a) {
+                           const int4 *__restrict__ i0a,
+                           const int32_t *__restrict__ species = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int t = base + tid;
   int flag = 0;
@@ -578,7 +580,10 @@ __device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0,
     const int oy = ox ? overlap(i0.y, g.ns, g.n[1], y0, tyw, &ky, &yl) : 0;
     if (ox && oy) {
       flag = 1;
-      r = int4{j, kx | (xl << 8) | (ox << 16), ky | (yl << 8) | (oy << 16), wrapi(i0.z, g.n[2])};
+      // rec.w: first z point (wrapped, < 2^16) | species << 16 when asked for
+      const int sp = species ? species[j] : 0;
+      r = int4{j, kx | (xl << 8) | (ox << 16), ky | (yl << 8) | (oy << 16),
+               wrapi(i0.z, g.n[2]) | (sp << 16)};
     }
   }
   const unsigned long long m = __ballot(flag);
@@ -604,7 +609,7 @@ __device__ void slab_lists(Stage &st, const int4 *rec, unsigned short *slist, in
   const int s = first + tid;
   unsigned mask = 0;
   if (s < last) {
-    const int a = rec[s].w;
+    const int a = rec[s].w & 0xFFFF;
     for (int w = 0; w < NW; ++w) {
       const int z0 = (n3 * w) / NW, z1 = (n3 * (w + 1)) / NW;
       const bool hit = (max(a, z0) < min(a + ns, z1)) || (a + ns > n3 && z0 < a + ns - n3);
@@ -702,7 +707,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int ns = g.ns;
   for (int base = 0; base < st.total; base += NT) {
     const int before = st.nstaged;
-    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a);
+    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a,
+                                g.F == 1 ? species : nullptr);
     slab_lists(st, rec, slist, before, nst, n3, ns);
     const bool last = base + NT >= st.total;
     if (!last && nst <= kStageCap - NT) continue;
@@ -718,10 +724,10 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
       if (t + 1 < cnt && lane < 3 * ns) ph_next = phi[(size_t)rec[mine[t + 1]].x * 3 * ns + lane];
       const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
       const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
-      const int a = r.w;
+      const int a = r.w & 0xFFFF;
       // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
       // (ox <= 16 and oy <= 2 by the tile choice, so ox*oy <= 64)
-      const T w = (g.F == 1) ? (T)g.amp[species[j]] : T(1);
+      const T w = (g.F == 1) ? (T)g.amp[r.w >> 16] : T(1);
       T fxy;
       int cofs;
       {
@@ -1298,7 +1304,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
       const int4 r = rec[s0 + min(grp, na - 1)];
       const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
       const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
-      const int a = r.w;
+      const int a = r.w & 0xFFFF;
       const bool cell = act && l < ox * oy;
       const int lc = cell ? l : 0;
       const int iy = small_div(lc, __builtin_amdgcn_rcpf((float)ox));
