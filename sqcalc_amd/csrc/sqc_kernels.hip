@@ -1257,16 +1257,18 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   struct VV {
     V a, b;
   };
+  const float inv_pw = 1.0f / (float)pw, inv_ty = 1.0f / (float)tyw;
   batched_copy<4, VV>(
       (M3 + 1) * tyw * pw,
       [&](int idx) {
-        const int p = idx % pw, r = idx / pw;
-        const V *q = src + ((size_t)(r / tyw) * n2 + y0 + r % tyw) * n1 + x0 + 2 * p;
+        const int r = small_div(idx, inv_pw), p = idx - r * pw;
+        const int k3 = small_div(r, inv_ty), yl = r - k3 * tyw;
+        const V *q = src + (unsigned)((k3 * n2 + y0 + yl) * n1 + x0 + 2 * p);
         return VV{q[0], q[1]};
       },
       [&](int idx, VV v) {
-        const int p = idx % pw, r = idx / pw;
-        const int yl = r % tyw, k3 = r / tyw;
+        const int r = small_div(idx, inv_pw), p = idx - r * pw;
+        const int k3 = small_div(r, inv_ty), yl = r - k3 * tyw;
         const int col = yl * ccx + p;
         const V A = v.a, B = v.b;
         tile[(size_t)col * cs + (k3)] = V{A.x - B.y, A.y + B.x};
