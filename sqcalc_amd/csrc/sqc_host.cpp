@@ -275,7 +275,7 @@ size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
   const sqc::Geo &g = p.geo;
   // the tile and the staged atom records; the DFT twiddles are read from
   // global memory (L1-resident), which leaves room for more workgroups
-  return (size_t)(txr / 2) * ty * g.cs * vsize(p) + sqc::kStageCap * (16 + 2 * 4);
+  return (size_t)(txr / 2) * ty * g.cs * vsize(p) + sqc::kStageCap * 16;
 }
 
 // y-line stride: the transposed copies put lane (line, y) at line * ls2 + y
@@ -399,7 +399,7 @@ void drop_graph(sqc::Plan &p) {
 
 void free_plan(sqc_hip_plan *h) {
   sqc::Plan &p = h->p;
-  void *bufs[] = {p.d_pos, p.d_X, p.d_i0, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
+  void *bufs[] = {p.d_pos, p.d_X, p.d_i0, p.d_i0s, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
                   p.d_species, p.d_W1, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
                   p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
                   p.d_line_off, p.d_line_d23, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
@@ -669,6 +669,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if ((rc = alloc(&p.d_pos, sizeof(double) * 3 * N * p.batch_cap, p.bytes))) return rc;
   if ((rc = alloc(&p.d_X, sizeof(double) * 3 * N, p.bytes))) return rc;
   if ((rc = alloc(&p.d_i0, sizeof(int4) * N, p.bytes))) return rc;
+  if ((rc = alloc(&p.d_i0s, sizeof(int4) * N, p.bytes))) return rc;
   if ((rc = alloc((char **)&p.d_phi, (vs / 2) * 3 * ns * N, p.bytes))) return rc;
   if ((rc = alloc((char **)&p.d_dphi, (vs / 2) * 3 * ns * N, p.bytes))) return rc;
   if ((rc = alloc(&p.d_key, sizeof(int32_t) * N, p.bytes))) return rc;

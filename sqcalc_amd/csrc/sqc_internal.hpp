@@ -120,6 +120,7 @@ struct Plan {
   double *d_pos = nullptr;
   double *d_X = nullptr;
   int4 *d_i0 = nullptr;        // stencil start per atom (i0x, i0y, i0z, -)
+  int4 *d_i0s = nullptr;       // the same in sorted (bin) order
   void *d_phi = nullptr;       // kernel values per atom [3][ns], path precision
   void *d_dphi = nullptr;      // kernel derivatives (adjoint)
   int32_t *d_key = nullptr, *d_count = nullptr, *d_start = nullptr, *d_cursor = nullptr,

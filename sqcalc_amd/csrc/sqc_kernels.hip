@@ -464,13 +464,14 @@ __global__ void k_scatter(int64_t natoms, const int32_t *__restrict__ key, int32
 
 // Order every bin by atom index: one wave per bin; bins of up to 64 atoms
 // are ranked in registers, larger ones fall back to an insertion sort.
-__global__ void k_binsort(int nb, const int32_t *__restrict__ start, int32_t *__restrict__ sidx) {
+__global__ void k_binsort(int nb, const int32_t *__restrict__ start, int32_t *__restrict__ sidx,
+                          const int4 *__restrict__ i0, int4 *__restrict__ i0s) {
   const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= nb) return;
   const int k0 = start[b], k1 = start[b + 1];
   const int n = k1 - k0;
-  if (n <= 1) return;
+  if (n == 0) return;
   if (n <= 64) {
     const int v = lane < n ? sidx[k0 + lane] : 0x7fffffff;
     int rank = 0;
@@ -478,19 +479,26 @@ __global__ void k_binsort(int nb, const int32_t *__restrict__ start, int32_t *__
       const int w = __shfl(v, i);
       rank += (w < v) ? 1 : 0;
     }
-    if (lane < n) sidx[k0 + rank] = v;
+    if (lane < n) {
+      sidx[k0 + rank] = v;
+      i0s[k0 + rank] = i0[v];  // the stencil corners in sorted order
+    }
     return;
   }
-  if (lane != 0) return;
-  for (int k = k0 + 1; k < k1; ++k) {
-    const int v = sidx[k];
-    int i = k - 1;
-    while (i >= k0 && sidx[i] > v) {
-      sidx[i + 1] = sidx[i];
-      --i;
+  if (lane == 0) {
+    for (int k = k0 + 1; k < k1; ++k) {
+      const int v = sidx[k];
+      int i = k - 1;
+      while (i >= k0 && sidx[i] > v) {
+        sidx[i + 1] = sidx[i];
+        --i;
+      }
+      sidx[i + 1] = v;
     }
-    sidx[i + 1] = v;
   }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  for (int k = k0 + lane; k < k1; k += 64) i0s[k] = i0[sidx[k]];
 }
 
 // Bins to visit around a tile, without visiting a bin twice on small grids.
@@ -510,7 +518,8 @@ __device__ __forceinline__ void bin_window(int t, int h, int nt, int *first, int
 // of the overlap flags), so every round is deterministic.
 //   rec.x = atom, rec.y = kx | xl << 8 | ox << 16, rec.z = ky | yl << 8 | oy << 16,
 //   rec.w = first z point of the stencil, wrapped into [0, n3), | species << 16
-//           (single-field spreads, which weight each atom)
+//           (single-field spreads, which weight each atom) | z-slab mask << 24
+//           (the spreader)
 // ---------------------------------------------------------------------------
 constexpr int kMaxWindowBins = 128;
 
@@ -521,8 +530,6 @@ struct Stage {
   int bin_pref[kMaxWindowBins + 1];   // candidates before each bin
   int wsum[NW];
   int nstaged;
-  int slab_n[NW];            // staged atoms that reach each wave's z slab
-  int slab_w[NW][NW];        // per-wave counts of one chunk, by slab
 };
 
 __device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
@@ -541,16 +548,27 @@ __device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
     st.bin_pref[t + 1] = start[bin + 1] - s0;  // counts, scanned below
   }
   __syncthreads();
-  if (t == 0) {
-    int tot = 0;
-    st.bin_pref[0] = 0;
-    for (int b = 0; b < nb; ++b) {
-      tot += st.bin_pref[b + 1];
-      st.bin_pref[b + 1] = tot;
+  if (t < 64) {
+    // inclusive scan of the counts by one wave, two bins per lane
+    // (nb <= kMaxWindowBins = 128)
+    const int b0 = 2 * t, b1 = b0 + 1;
+    const int c0 = b0 < nb ? st.bin_pref[b0 + 1] : 0;
+    const int c1 = b1 < nb ? st.bin_pref[b1 + 1] : 0;
+    int sum = c0 + c1;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(sum, off);
+      if (t >= off) sum += y;
     }
-    st.nbin = nb;
-    st.total = tot;
-    st.nstaged = 0;
+    const int excl = sum - c0 - c1;
+    if (b0 < nb) st.bin_pref[b0 + 1] = excl + c0;
+    if (b1 < nb) st.bin_pref[b1 + 1] = excl + c0 + c1;
+    if (t == 63) st.total = sum;
+    if (t == 0) {
+      st.bin_pref[0] = 0;
+      st.nbin = nb;
+      st.nstaged = 0;
+    }
   }
   __syncthreads();
 }
@@ -559,8 +577,8 @@ __device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
 // staged; returns (block-uniform) the number staged so far.
 __device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0, int txw, int y0,
                            int tyw, const int32_t *__restrict__ sidx,
-                           const int4 *__restrict__ i0a,
-                           const int32_t *__restrict__ species = nullptr) {
+                           const int4 *__restrict__ i0s,
+                           const int32_t *__restrict__ species = nullptr, int slabs = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int t = base + tid;
   int flag = 0;
@@ -573,17 +591,27 @@ __device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0,
       if (st.bin_pref[mid] <= t) b = mid;
       else hi = mid - 1;
     }
-    const int j = sidx[st.bin_first[b] + (t - st.bin_pref[b])];
-    const int4 i0 = i0a[j];
+    const int k = st.bin_first[b] + (t - st.bin_pref[b]);
+    const int j = sidx[k];
+    const int4 i0 = i0s[k];  // sorted copy: independent of the sidx load
     int kx = 0, xl = 0, ky = 0, yl = 0;
     const int ox = overlap(i0.x, g.ns, g.n[0], x0, txw, &kx, &xl);
     const int oy = ox ? overlap(i0.y, g.ns, g.n[1], y0, tyw, &ky, &yl) : 0;
     if (ox && oy) {
       flag = 1;
-      // rec.w: first z point (wrapped, < 2^16) | species << 16 when asked for
+      // rec.w: first z point (wrapped, < 2^16) | species << 16 when asked
+      // for | mask of the z slabs the stencil reaches << 24 when asked for
       const int sp = species ? species[j] : 0;
+      const int n3 = g.n[2], ns = g.ns;
+      const int a = wrapi(i0.z, n3);
+      int mask = 0;
+      for (int w = 0; w < slabs; ++w) {
+        const int z0 = (n3 * w) / slabs, z1 = (n3 * (w + 1)) / slabs;
+        const bool hit = (max(a, z0) < min(a + ns, z1)) || (a + ns > n3 && z0 < a + ns - n3);
+        mask |= hit ? (1 << w) : 0;
+      }
       r = int4{j, kx | (xl << 8) | (ox << 16), ky | (yl << 8) | (oy << 16),
-               wrapi(i0.z, g.n[2]) | (sp << 16)};
+               a | (sp << 16) | (mask << 24)};
     }
   }
   const unsigned long long m = __ballot(flag);
@@ -600,43 +628,6 @@ __device__ int stage_chunk(Stage &st, int4 *rec, int base, const Geo &g, int x0,
   return total;
 }
 
-// After stage_chunk: append the atoms of this chunk to the per-slab index
-// lists (slab w = z points [w n3 / NW, (w+1) n3 / NW)), in staging order.
-__device__ void slab_lists(Stage &st, const int4 *rec, unsigned short *slist, int first, int last,
-                           int n3, int ns) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // the staged atoms of this chunk are rec[first .. last); one per thread
-  const int s = first + tid;
-  unsigned mask = 0;
-  if (s < last) {
-    const int a = rec[s].w & 0xFFFF;
-    for (int w = 0; w < NW; ++w) {
-      const int z0 = (n3 * w) / NW, z1 = (n3 * (w + 1)) / NW;
-      const bool hit = (max(a, z0) < min(a + ns, z1)) || (a + ns > n3 && z0 < a + ns - n3);
-      if (hit) mask |= 1u << w;
-    }
-  }
-  unsigned long long m[NW];
-  for (int w = 0; w < NW; ++w) {
-    m[w] = __ballot((mask >> w) & 1u);
-    if (lane == 0) st.slab_w[wave][w] = __popcll(m[w]);
-  }
-  __syncthreads();
-  for (int w = 0; w < NW; ++w) {
-    if (!((mask >> w) & 1u)) continue;
-    int off = st.slab_n[w];
-    for (int v = 0; v < wave; ++v) off += st.slab_w[v][w];
-    off += __popcll(m[w] & ((1ULL << lane) - 1ULL));
-    slist[w * kStageCap + off] = (unsigned short)s;
-  }
-  __syncthreads();
-  if (tid < NW) {
-    int add = 0;
-    for (int v = 0; v < NW; ++v) add += st.slab_w[v][tid];
-    st.slab_n[tid] += add;
-  }
-  __syncthreads();
-}
 
 // Add the z points [z0, z0 + pc) of one atom's stencil into the tile: cell
 // c = xy * pc + kz, two cells per lane per round with both reads issued
@@ -676,7 +667,7 @@ __device__ __forceinline__ void spread_piece(T *tflat, T ph, T fxy, int cofs, in
 // wave, in atom order: no atomics, and the grid is bit-reproducible.
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 *__restrict__ i0a,
+__global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 *__restrict__ i0s,
                                                  const T *__restrict__ phi,
                                                  const int32_t *__restrict__ sidx,
                                                  const int32_t *__restrict__ start,
@@ -690,7 +681,6 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int ccx = g.txr / 2, cc = ccx * g.ty;
   V *tile = reinterpret_cast<V *>(smem);
   int4 *rec = reinterpret_cast<int4 *>(tile + (size_t)cc * cs);
-  unsigned short *slist = reinterpret_cast<unsigned short *>(rec + kStageCap);
   T *tflat = reinterpret_cast<T *>(tile);
   // grid (tiles, fields) in hardware order: measured faster here than the
   // XCD-contiguous renumbering the y pass uses
@@ -700,59 +690,73 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
-  if (tid < NW) st.slab_n[tid] = 0;
   stage_window(st, g, tx, ty, f, start);
 
   const int zs0 = (n3 * wave) / NW, zs1 = (n3 * (wave + 1)) / NW;
   const int ns = g.ns;
+  // spread one staged atom into this wave's slab; ph: lane l < 3 ns holds
+  // its kernel value l (x, then y, then z)
+  auto spread_atom = [&](const int4 r, const T ph) {
+    const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
+    const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
+    const int a = r.w & 0xFFFF;
+    // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
+    // (ox <= 16 and oy <= 2 by the tile choice, so ox*oy <= 64)
+    const T w = (g.F == 1) ? (T)g.amp[(r.w >> 16) & 0xFF] : T(1);
+    T fxy;
+    int cofs;
+    {
+      const int oxy = ox * oy;
+      const int l = lane < oxy ? lane : 0;
+      const int iy = small_div(l, __builtin_amdgcn_rcpf((float)ox));
+      const int ix = l - iy * ox;
+      fxy = w * __shfl(ph, kxl + ix) * __shfl(ph, ns + kyl + iy);
+      const int xl = xll + ix, yl = yll + iy;
+      cofs = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
+    }
+    // the stencil's z range inside this wave's slab: one piece, or two
+    // when it wraps through z = 0
+    const int e1 = min(a + ns, n3);
+    spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + max(a, zs0) - a, max(a, zs0),
+                    min(e1, zs1) - max(a, zs0), lane);
+    if (a + ns > n3) {
+      const int lo2 = zs0, hi2 = min(a + ns - n3, zs1);
+      spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + n3 - a + lo2, lo2, hi2 - lo2, lane);
+    }
+  };
+
   for (int base = 0; base < st.total; base += NT) {
-    const int before = st.nstaged;
-    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a,
-                                g.F == 1 ? species : nullptr);
-    slab_lists(st, rec, slist, before, nst, n3, ns);
-    const bool last = base + NT >= st.total;
-    if (!last && nst <= kStageCap - NT) continue;
-    // spread the atoms of this wave's slab, in staging order
-    const unsigned short *mine = slist + wave * kStageCap;
-    const int cnt = st.slab_n[wave];
-    T ph_next = (cnt > 0 && lane < 3 * ns) ? phi[(size_t)rec[mine[0]].x * 3 * ns + lane] : T(0);
-    for (int t = 0; t < cnt; ++t) {
-      const int4 r = rec[mine[t]];
-      const int j = r.x;
-      // the kernel values of this atom were loaded one iteration ahead
-      const T ph = ph_next;
-      if (t + 1 < cnt && lane < 3 * ns) ph_next = phi[(size_t)rec[mine[t + 1]].x * 3 * ns + lane];
-      const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
-      const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
-      const int a = r.w & 0xFFFF;
-      // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
-      // (ox <= 16 and oy <= 2 by the tile choice, so ox*oy <= 64)
-      const T w = (g.F == 1) ? (T)g.amp[r.w >> 16] : T(1);
-      T fxy;
-      int cofs;
-      {
-        const int oxy = ox * oy;
-        const int l = lane < oxy ? lane : 0;
-        const int iy = small_div(l, __builtin_amdgcn_rcpf((float)ox));
-        const int ix = l - iy * ox;
-        fxy = w * __shfl(ph, kxl + ix) * __shfl(ph, ns + kyl + iy);
-        const int xl = xll + ix, yl = yll + iy;
-        cofs = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
-      }
-      // the stencil's z range inside this wave's slab: one piece, or two
-      // when it wraps through z = 0
-      const int e1 = min(a + ns, n3);
-      spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + max(a, zs0) - a, max(a, zs0),
-                      min(e1, zs1) - max(a, zs0), lane);
-      if (a + ns > n3) {
-        const int lo2 = zs0, hi2 = min(a + ns - n3, zs1);
-        spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + n3 - a + lo2, lo2, hi2 - lo2,
-                        lane);
+    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0s,
+                                g.F == 1 ? species : nullptr, NW);
+    // this wave's atoms in staging order: a ballot on its slab bit per 64
+    // records, then the set bits in turn; the next atom's record and kernel
+    // values are read while the current one is spread
+    for (int b0 = 0; b0 < nst; b0 += 64) {
+      const int idx = b0 + lane;
+      const int wv = idx < nst ? rec[idx].w : 0;
+      unsigned long long m = __ballot((wv >> (24 + wave)) & 1);
+      if (!m) continue;
+      int u = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      int4 rc = rec[b0 + u];
+      T pc = lane < 3 * ns ? phi[(size_t)rc.x * 3 * ns + lane] : T(0);
+      while (u >= 0) {
+        const int un = m ? __ffsll((long long)m) - 1 : -1;
+        int4 rn = rc;
+        T pn = pc;
+        if (un >= 0) {
+          m &= m - 1;
+          rn = rec[b0 + un];
+          if (lane < 3 * ns) pn = phi[(size_t)rn.x * 3 * ns + lane];
+        }
+        spread_atom(rc, pc);
+        u = un;
+        rc = rn;
+        pc = pn;
       }
     }
-    __syncthreads();
+    __syncthreads();  // the next chunk overwrites rec from its start
     if (tid == 0) st.nstaged = 0;
-    if (tid < NW) st.slab_n[tid] = 0;
     __syncthreads();
   }
   __syncthreads();
@@ -1231,6 +1235,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
                                                      const typename C2<T>::t *__restrict__ W1,
                                                      const typename C2<T>::t *__restrict__ twz,
                                                      const int4 *__restrict__ i0a,
+                                                     const int4 *__restrict__ i0s,
                                                      const T *__restrict__ phi,
                                                      const T *__restrict__ dphi,
                                                      const int32_t *__restrict__ sidx,
@@ -1290,7 +1295,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   const int grp = lane / lpa, l = lane - grp * lpa;
   T *scr = reinterpret_cast<T *>(rec + kStageCap) + (size_t)wave * ga * 6 * ns;
   for (int base = 0; base < st.total; base += NT) {
-    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0a);
+    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0s);
     const bool last = base + NT >= st.total;
     if (!last && nst <= kStageCap - NT) continue;
     for (int s0 = wave * ga; s0 < nst; s0 += NW * ga) {
@@ -1495,7 +1500,7 @@ static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
   hipLaunchKernelGGL(k_scatter, dim3(ga), dim3(blk), 0, st, (int64_t)g.natoms, p.d_key,
                      p.d_cursor, p.d_sidx);
   hipLaunchKernelGGL(k_binsort, dim3((nbins_tot + 3) / 4), dim3(256), 0, st, nbins_tot, p.d_start,
-                     p.d_sidx);
+                     p.d_sidx, p.d_i0, p.d_i0s);
 }
 
 template <typename T>
@@ -1508,7 +1513,7 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
     StageTimer t(p, 1);
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
       hipLaunchKernelGGL((k_spread_z<T, decltype(n)::value>), dim3(g.ntx * g.nty, g.F), dim3(NT),
-                         p.lds1, st, g, p.ax[2], p.d_i0, (const T *)p.d_phi, p.d_sidx, p.d_start,
+                         p.lds1, st, g, p.ax[2], p.d_i0s, (const T *)p.d_phi, p.d_sidx, p.d_start,
                          p.d_species, (const V *)p.d_tw[2], (V *)p.d_W1);
     });
   }
@@ -1587,7 +1592,8 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
       hipLaunchKernelGGL((k_adj_z_gather<T, decltype(n)::value>), dim3(g.ntx * g.nty, Fa),
                          dim3(NT), p.lds1g, st, g, p.ax[2], (const V *)p.d_W1,
-                         (const V *)p.d_tw[2], p.d_i0, (const T *)p.d_phi, (const T *)p.d_dphi,
+                         (const V *)p.d_tw[2], p.d_i0, p.d_i0s, (const T *)p.d_phi,
+                         (const T *)p.d_dphi,
                          p.d_sidx, p.d_start, p.d_slots);
     });
   }

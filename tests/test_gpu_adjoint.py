@@ -65,3 +65,39 @@ def test_adjoint_matches_oracle(hip_device, precision, weight, partials):
     print("  max |dg| / max |g| = %.2e" % err)
     assert err <= TOL[precision]
     sf.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_adjoint_dense_windows(hip_device, precision):
+    """Gather windows that span many staging rounds (6,000 atoms in 12^3 A^3).
+    FP64 pins the gather to 1e-8; in FP32 the tiny dense grid amplifies
+    rounding ~N / sqrt(N) (see the dense forward test), bound 1e-4."""
+    import torch
+
+    from sqcalc_amd import HipStructureFactor
+    from sqcalc_amd.autograd import structure_factor
+
+    cell, origin, types, frames = random_case(natoms=6000, length=12.0, nframes=1, seed=52)
+    print("case adjoint dense", precision, flush=True)
+    qmax, nq = 2.5, 25
+    eps = 1e-11 if precision == "double" else 1e-6
+    sf = HipStructureFactor(cell, origin, types, {1: "Si", 2: "O"}, weight="neutron",
+                            norm="mean", qmax=qmax, nq=nq, precision=precision, partials=False,
+                            eps=eps)
+    rng = np.random.default_rng(9)
+    target = torch.tensor(rng.random(nq), device=hip_device)
+    pos = torch.tensor(frames[0], device=hip_device, requires_grad=True)
+    S, _ = structure_factor(pos, sf)
+    torch.sum((S - target) ** 2).backward()
+    grad = pos.grad.cpu().numpy()
+    run = O.Run(cell, origin, types, {1: "Si", 2: "O"}, weight="neutron", norm="mean",
+                qmax=qmax, nq=nq, partials=False)
+    rho = O.direct_rho(frames[0], cell, origin, run.species_of, 2, run.md, method=1)
+    num, _ = O.shell_sums(rho, run.md, "neutron", run.amp, run.xcoef)
+    Sref = np.where(run.den > 0, num / np.where(run.den > 0, run.den, 1), 0)
+    gref = O.gradient(frames[0], cell, origin, run.species_of, run.md, rho, "neutron", run.amp,
+                      run.xcoef, 2 * (Sref - target.cpu().numpy()), run.den, run.shell_count)
+    err = np.abs(grad - gref).max() / np.abs(gref).max()
+    print("  max |dg| / max |g| = %.2e" % err)
+    assert err <= (TOL[precision] if precision == "double" else 1e-4)
+    sf.close()
