@@ -278,14 +278,15 @@ size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
   return (size_t)(txr / 2) * ty * g.cs * vsize(p) + sqc::kStageCap * 16;
 }
 
-// y-line stride: the transposed copies put lane (line, y) at line * ls2 + y
-// (line fastest, tx2 lines); with ls2 = 32 / tx2 mod 32 elements (FP32) or
-// 16 / tx2 mod 16 (FP64) the lanes of a half wave hit distinct banks.
+// y-line stride: the smallest odd stride that holds a line (n2 + n2/32 + 1
+// points, the DFT's padded stage layout).  An odd stride spreads the
+// transposed copies (lane (line, y) at line * ls2 + y) over the banks and
+// keeps five 8-line workgroups per CU at n2 = 480; strides tuned to the bank
+// count measured no faster.
 int ystride(const sqc::Plan &p, int tx2) {
-  const int n2 = p.geo.n[1];
-  const int period = p.prec == 1 ? 32 : 16;
-  const int want = (period / std::min(tx2, period)) % period;
-  return n2 + ((want - n2) % period + period) % period;
+  (void)tx2;
+  const int cap = p.geo.n[1] + p.geo.n[1] / 32 + 1;
+  return cap | 1;
 }
 
 size_t ypass_lds(const sqc::Plan &p, int tx2) {
@@ -616,8 +617,10 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
                                              std::to_string(sqc::kMaxLine));
   // LDS z column stride of the spread tile: 24 words mod 64 spreads the
   // neighbouring columns of a spread update over the banks
-  g.cs = n3 + ((12 - n3) % 32 + 32) % 32;
-  g.rs1 = n1;
+  // (line buffers hold n + n/32 + 1 points: the DFT's padded stage layout)
+  const int cap3 = n3 + n3 / 32 + 1;
+  g.cs = cap3 + ((12 - cap3) % 32 + 32) % 32;
+  g.rs1 = n1 + n1 / 32 + 1;
   {
     int pick = -1;
     for (int pass = 0; pass < 2 && pick < 0; ++pass)

@@ -219,21 +219,36 @@ constexpr RadixPlan radix_plan(int n) {
   return p;
 }
 
-template <typename T, int N, int R, int Ns>
+// The first stage writes element e = R j + r (stride R across the lanes),
+// which falls on few LDS banks.  When R divides 32 and the second stage reads
+// with a stride that is a multiple of 32, the two stages exchange the data in
+// a padded layout, e -> e + e / 32: both index maps stay a per-lane base plus
+// compile-time offsets, so the padding costs no instructions.  Line buffers
+// hold N + N / 32 + 1 elements.
+template <int N> struct StagePad {
+  static constexpr RadixPlan P = radix_plan(N);
+  static constexpr bool on =
+      P.nst >= 2 && 32 % P.rad[0] == 0 && (N / P.rad[1]) % 32 == 0;
+};
+
+template <typename T, int N, int R, int Ns, bool InPad, bool OutPad>
 __device__ __forceinline__ void fixed_stage(typename C2<T>::t *line, const typename C2<T>::t *tw,
                                             int lane) {
   using V = typename C2<T>::t;
   constexpr int nb = N / R;
   constexpr int MAXB = (nb + 63) / 64;
   constexpr int tstep = N / (Ns * R);
+  constexpr int rstride = InPad ? nb + nb / 32 : nb;
+  static_assert(!InPad || nb % 32 == 0, "padded reads need a stride that is a multiple of 32");
+  static_assert(!OutPad || (Ns == 1 && 32 % R == 0), "padded writes are for a first stage");
   V v[MAXB][R];
 #pragma unroll
   for (int it = 0; it < MAXB; ++it) {
     const int j = lane + 64 * it;
     if (it + 1 < MAXB || nb % 64 == 0 || j < nb) {
-      const V *src = line + j;
+      const V *src = line + j + (InPad ? (j >> 5) : 0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = src[r * nb];
+      for (int r = 0; r < R; ++r) v[it][r] = src[r * rstride];
       if constexpr (Ns > 1) {
         const int k = j % Ns;
 #pragma unroll
@@ -247,9 +262,15 @@ __device__ __forceinline__ void fixed_stage(typename C2<T>::t *line, const typen
     const int j = lane + 64 * it;
     if (it + 1 < MAXB || nb % 64 == 0 || j < nb) {
       Dft<T, R>::run(v[it]);
-      V *dst = line + (j / Ns) * (Ns * R) + j % Ns;
+      if constexpr (OutPad) {
+        V *dst = line + j * R + ((j * R) >> 5);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * Ns] = v[it][r];
+        for (int r = 0; r < R; ++r) dst[r] = v[it][r];
+      } else {
+        V *dst = line + (j / Ns) * (Ns * R) + j % Ns;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * Ns] = v[it][r];
+      }
     }
   }
   wave_sync();
@@ -261,7 +282,8 @@ __device__ __forceinline__ void fixed_stages(typename C2<T>::t *line, const type
   constexpr RadixPlan P = radix_plan(N);
   if constexpr (S < P.nst) {
     constexpr int R = P.rad[S];
-    fixed_stage<T, N, R, Ns>(line, tw, lane);
+    constexpr bool pad = StagePad<N>::on;
+    fixed_stage<T, N, R, Ns, pad && S == 1, pad && S == 0>(line, tw, lane);
     fixed_stages<T, N, S + 1, Ns * R>(line, tw, lane);
   }
 }
