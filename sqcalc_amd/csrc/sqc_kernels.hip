@@ -231,9 +231,11 @@ template <int N> struct StagePad {
       P.nst >= 2 && 32 % P.rad[0] == 0 && (N / P.rad[1]) % 32 == 0;
 };
 
-template <typename T, int N, int R, int Ns, bool InPad, bool OutPad>
-__device__ __forceinline__ void fixed_stage(typename C2<T>::t *line, const typename C2<T>::t *tw,
-                                            int lane) {
+// One stage over L lines of the same length at once: the lines' LDS reads
+// are issued together and each twiddle is read once for all of them.
+template <typename T, int N, int R, int Ns, bool InPad, bool OutPad, int L>
+__device__ __forceinline__ void fixed_stage(typename C2<T>::t *const *ln,
+                                            const typename C2<T>::t *tw, int lane) {
   using V = typename C2<T>::t;
   constexpr int nb = N / R;
   constexpr int MAXB = (nb + 63) / 64;
@@ -241,18 +243,25 @@ __device__ __forceinline__ void fixed_stage(typename C2<T>::t *line, const typen
   constexpr int rstride = InPad ? nb + nb / 32 : nb;
   static_assert(!InPad || nb % 32 == 0, "padded reads need a stride that is a multiple of 32");
   static_assert(!OutPad || (Ns == 1 && 32 % R == 0), "padded writes are for a first stage");
-  V v[MAXB][R];
+  V v[L][MAXB][R];
 #pragma unroll
   for (int it = 0; it < MAXB; ++it) {
     const int j = lane + 64 * it;
     if (it + 1 < MAXB || nb % 64 == 0 || j < nb) {
-      const V *src = line + j + (InPad ? (j >> 5) : 0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = src[r * rstride];
+      for (int l = 0; l < L; ++l) {
+        const V *src = ln[l] + j + (InPad ? (j >> 5) : 0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[l][it][r] = src[r * rstride];
+      }
       if constexpr (Ns > 1) {
         const int k = j % Ns;
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[it][r] = cmul(v[it][r], tw[k * (r * tstep)]);
+        for (int r = 1; r < R; ++r) {
+          const V w = tw[k * (r * tstep)];
+#pragma unroll
+          for (int l = 0; l < L; ++l) v[l][it][r] = cmul(v[l][it][r], w);
+        }
       }
     }
   }
@@ -261,30 +270,33 @@ __device__ __forceinline__ void fixed_stage(typename C2<T>::t *line, const typen
   for (int it = 0; it < MAXB; ++it) {
     const int j = lane + 64 * it;
     if (it + 1 < MAXB || nb % 64 == 0 || j < nb) {
-      Dft<T, R>::run(v[it]);
-      if constexpr (OutPad) {
-        V *dst = line + j * R + ((j * R) >> 5);
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[r] = v[it][r];
-      } else {
-        V *dst = line + (j / Ns) * (Ns * R) + j % Ns;
+      for (int l = 0; l < L; ++l) {
+        Dft<T, R>::run(v[l][it]);
+        if constexpr (OutPad) {
+          V *dst = ln[l] + j * R + ((j * R) >> 5);
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[r * Ns] = v[it][r];
+          for (int r = 0; r < R; ++r) dst[r] = v[l][it][r];
+        } else {
+          V *dst = ln[l] + (j / Ns) * (Ns * R) + j % Ns;
+#pragma unroll
+          for (int r = 0; r < R; ++r) dst[r * Ns] = v[l][it][r];
+        }
       }
     }
   }
   wave_sync();
 }
 
-template <typename T, int N, int S, int Ns>
-__device__ __forceinline__ void fixed_stages(typename C2<T>::t *line, const typename C2<T>::t *tw,
-                                             int lane) {
+template <typename T, int N, int S, int Ns, int L>
+__device__ __forceinline__ void fixed_stages(typename C2<T>::t *const *ln,
+                                             const typename C2<T>::t *tw, int lane) {
   constexpr RadixPlan P = radix_plan(N);
   if constexpr (S < P.nst) {
     constexpr int R = P.rad[S];
     constexpr bool pad = StagePad<N>::on;
-    fixed_stage<T, N, R, Ns, pad && S == 1, pad && S == 0>(line, tw, lane);
-    fixed_stages<T, N, S + 1, Ns * R>(line, tw, lane);
+    fixed_stage<T, N, R, Ns, pad && S == 1, pad && S == 0, L>(ln, tw, lane);
+    fixed_stages<T, N, S + 1, Ns * R, L>(ln, tw, lane);
   }
 }
 
@@ -293,10 +305,28 @@ __device__ __forceinline__ void fixed_stages(typename C2<T>::t *line, const type
 template <typename T, int N>
 __device__ __forceinline__ void wave_fft(typename C2<T>::t *line, const AxisPlan &ax,
                                          const typename C2<T>::t *tw, int lane) {
-  if constexpr (N > 0)
-    fixed_stages<T, N, 0, 1>(line, tw, lane);
-  else
+  if constexpr (N > 0) {
+    typename C2<T>::t *ln[1] = {line};
+    fixed_stages<T, N, 0, 1, 1>(ln, tw, lane);
+  } else {
     wave_fft_any<T>(line, ax, tw, lane);
+  }
+}
+
+// The DFTs of lines first, first + NW, ... < nlines (lines at base + i *
+// stride) by one wave.  FP32 lines of a compile-time length go two at a time.
+template <typename T, int N>
+__device__ __forceinline__ void wave_fft_lines(typename C2<T>::t *base, int stride, int first,
+                                               int nlines, const AxisPlan &ax,
+                                               const typename C2<T>::t *tw, int lane) {
+  int i = first;
+  if constexpr (N > 0 && sizeof(T) == 4) {
+    for (; i + NW < nlines; i += 2 * NW) {
+      typename C2<T>::t *ln[2] = {base + (size_t)i * stride, base + (size_t)(i + NW) * stride};
+      fixed_stages<T, N, 0, 1, 2>(ln, tw, lane);
+    }
+  }
+  for (; i < nlines; i += NW) wave_fft<T, N>(base + (size_t)i * stride, ax, tw, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -680,6 +710,64 @@ __device__ __forceinline__ void spread_piece(T *tflat, T ph, T fxy, int cofs, in
   }
 }
 
+// Lane shuffle through ds_bpermute with a byte address (4 * source lane)
+// computed once by the caller.
+__device__ __forceinline__ int bperm(int addr4, int v) { return __builtin_amdgcn_ds_bpermute(addr4, v); }
+__device__ __forceinline__ float bperm(int addr4, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr4, __float_as_int(v)));
+}
+__device__ __forceinline__ double bperm(int addr4, double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr4, (int)b);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr4, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Cells of a whole stencil (all ns z points in the wave's slab) go through a
+// map fixed per lane: cell c = lane + 64 h is xy cell c / ns, z point c % ns.
+// Up to kWholeRounds rounds of 64 cells (ox * oy * ns <= 192).
+constexpr int kWholeRounds = 3;
+
+struct WholeMap {
+  int xy4[kWholeRounds];   // 4 * xy cell (bpermute address of fxy and cofs)
+  int fz4[kWholeRounds];   // 4 * (2 ns + z point) (bpermute address of fz in ph)
+  int z2[kWholeRounds];    // 2 * z point (offset in the interleaved tile)
+};
+
+__device__ __forceinline__ WholeMap whole_map(int ns, int lane) {
+  WholeMap m;
+  const float inv_ns = __builtin_amdgcn_rcpf((float)ns);
+#pragma unroll
+  for (int h = 0; h < kWholeRounds; ++h) {
+    const int c = lane + 64 * h;
+    const int xy = small_div(c, inv_ns), z = c - xy * ns;
+    m.xy4[h] = 4 * xy;
+    m.fz4[h] = 4 * (2 * ns + z);
+    m.z2[h] = 2 * z;
+  }
+  return m;
+}
+
+// Add a whole stencil starting at z point a into the tile: all reads are
+// issued before the writes (the cells of one atom are distinct).
+template <typename T>
+__device__ __forceinline__ void spread_whole(T *tflat, T ph, T fxy, int cofs, int cells, int a,
+                                             const WholeMap &m, int lane) {
+  T f[kWholeRounds], r[kWholeRounds];
+  int o[kWholeRounds];
+#pragma unroll
+  for (int h = 0; h < kWholeRounds; ++h) {
+    if (64 * h < cells) {
+      f[h] = bperm(m.xy4[h], fxy) * bperm(m.fz4[h], ph);
+      o[h] = bperm(m.xy4[h], cofs) + 2 * a + m.z2[h];
+      r[h] = lane + 64 * h < cells ? tflat[o[h]] : T(0);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < kWholeRounds; ++h)
+    if (64 * h < cells && lane + 64 * h < cells) tflat[o[h]] = r[h] + f[h];
+}
+
 // ---------------------------------------------------------------------------
 // Spread + z-DFT.  One workgroup per (x,y) tile of txr x ty real columns and
 // one field.  The tile is held in LDS as txr/2 x ty complex columns of n3
@@ -716,6 +804,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
 
   const int zs0 = (n3 * wave) / NW, zs1 = (n3 * (wave + 1)) / NW;
   const int ns = g.ns;
+  const WholeMap wm = whole_map(ns, lane);
   // spread one staged atom into this wave's slab; ph: lane l < 3 ns holds
   // its kernel value l (x, then y, then z)
   auto spread_atom = [&](const int4 r, const T ph) {
@@ -735,6 +824,11 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
       fxy = w * __shfl(ph, kxl + ix) * __shfl(ph, ns + kyl + iy);
       const int xl = xll + ix, yl = yll + iy;
       cofs = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
+    }
+    const int cells = ox * oy * ns;
+    if (a >= zs0 && a + ns <= zs1 && cells <= 64 * kWholeRounds) {
+      spread_whole<T>(tflat, ph, fxy, cofs, cells, a, wm, lane);
+      return;
     }
     // the stencil's z range inside this wave's slab: one piece, or two
     // when it wraps through z = 0
@@ -782,7 +876,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     __syncthreads();
   }
   __syncthreads();
-  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, twz, lane);
+  wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane);
   __syncthreads();
   // split the packed pairs and write h3 = 0..M3.  txr * ty is a power of two
   // that divides NT, so every thread keeps one (xl, yl) column and steps h3.
@@ -837,7 +931,7 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
       },
       [&](int idx, V v) { lines[(idx & tm) * ls + (idx >> lt)] = v; });
   __syncthreads();
-  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, twy, lane);
+  wave_fft_lines<T, N>(lines, ls, wave, tx2, ay, twy, lane);
   __syncthreads();
   const int nl = hi - lo + 1;
   V *dst = W2 + ((size_t)f * g.nlines + loff[k3]) * n1;
@@ -975,6 +1069,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
 #pragma unroll
       for (int k = 0; k < kPreShell; ++k) sv[k] = stab[min(c0 + tid + k * NT, c1 - 1)];
     }
+    // one line at a time: pairs raise the register count past three waves per SIMD
     for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
     __syncthreads();
     auto bin_mode = [&](int c, unsigned s) {
@@ -1237,7 +1332,7 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
         if (x0 + (idx & tm) < n1) lines[(idx & tm) * ls + yi] = v;
       });
   __syncthreads();
-  for (int l = wave; l < tx2; l += NW) wave_fft<T, N>(lines + (size_t)l * ls, ay, twy, lane);
+  wave_fft_lines<T, N>(lines, ls, wave, tx2, ay, twy, lane);
   __syncthreads();
   V *dst = W1 + ((size_t)cf * (M3 + 1) + k3) * n2 * n1;
   for (int idx = tid; idx < n2 * tx2; idx += NT) {
@@ -1302,7 +1397,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         if (k3 > 0) tile[(size_t)col * cs + (n3 - k3)] = V{A.x + B.y, B.x - A.y};
       });
   __syncthreads();
-  for (int col = wave; col < cc; col += NW) wave_fft<T, N>(tile + (size_t)col * cs, az, twz, lane);
+  wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane);
   stage_window(st, g, tx, ty, cf, start);
 
   const int ns = g.ns;
