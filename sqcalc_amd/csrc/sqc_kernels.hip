@@ -727,6 +727,8 @@ __device__ __forceinline__ double bperm(int addr4, double v) {
 // map fixed per lane: cell c = lane + 64 h is xy cell c / ns, z point c % ns.
 // Up to kWholeRounds rounds of 64 cells (ox * oy * ns <= 192).
 constexpr int kWholeRounds = 3;
+// Registers of kernel values a spreading wave loads at once.
+constexpr int kPhiRegs = 4;
 
 struct WholeMap {
   int xy4[kWholeRounds];   // 4 * xy cell (bpermute address of fxy and cofs)
@@ -752,13 +754,13 @@ __device__ __forceinline__ WholeMap whole_map(int ns, int lane) {
 // issued before the writes (the cells of one atom are distinct).
 template <typename T>
 __device__ __forceinline__ void spread_whole(T *tflat, T ph, T fxy, int cofs, int cells, int a,
-                                             const WholeMap &m, int lane) {
+                                             int po4, const WholeMap &m, int lane) {
   T f[kWholeRounds], r[kWholeRounds];
   int o[kWholeRounds];
 #pragma unroll
   for (int h = 0; h < kWholeRounds; ++h) {
     if (64 * h < cells) {
-      f[h] = bperm(m.xy4[h], fxy) * bperm(m.fz4[h], ph);
+      f[h] = bperm(m.xy4[h], fxy) * bperm(m.fz4[h] + po4, ph);
       o[h] = bperm(m.xy4[h], cofs) + 2 * a + m.z2[h];
       r[h] = lane + 64 * h < cells ? tflat[o[h]] : T(0);
     }
@@ -807,7 +809,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const WholeMap wm = whole_map(ns, lane);
   // spread one staged atom into this wave's slab; ph: lane l < 3 ns holds
   // its kernel value l (x, then y, then z)
-  auto spread_atom = [&](const int4 r, const T ph) {
+  // (lanes po .. po + 3 ns - 1 of ph: po is 0 or 32)
+  auto spread_atom = [&](const int4 r, const T ph, const int po) {
     const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
     const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
     const int a = r.w & 0xFFFF;
@@ -821,23 +824,24 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
       const int l = lane < oxy ? lane : 0;
       const int iy = small_div(l, __builtin_amdgcn_rcpf((float)ox));
       const int ix = l - iy * ox;
-      fxy = w * __shfl(ph, kxl + ix) * __shfl(ph, ns + kyl + iy);
+      fxy = w * __shfl(ph, po + kxl + ix) * __shfl(ph, po + ns + kyl + iy);
       const int xl = xll + ix, yl = yll + iy;
       cofs = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
     }
     const int cells = ox * oy * ns;
     if (a >= zs0 && a + ns <= zs1 && cells <= 64 * kWholeRounds) {
-      spread_whole<T>(tflat, ph, fxy, cofs, cells, a, wm, lane);
+      spread_whole<T>(tflat, ph, fxy, cofs, cells, a, 4 * po, wm, lane);
       return;
     }
     // the stencil's z range inside this wave's slab: one piece, or two
     // when it wraps through z = 0
     const int e1 = min(a + ns, n3);
-    spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + max(a, zs0) - a, max(a, zs0),
+    spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, po + 2 * ns + max(a, zs0) - a, max(a, zs0),
                     min(e1, zs1) - max(a, zs0), lane);
     if (a + ns > n3) {
       const int lo2 = zs0, hi2 = min(a + ns - n3, zs1);
-      spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, 2 * ns + n3 - a + lo2, lo2, hi2 - lo2, lane);
+      spread_piece<T>(tflat, ph, fxy, cofs, ox * oy, po + 2 * ns + n3 - a + lo2, lo2, hi2 - lo2,
+                      lane);
     }
   };
 
@@ -845,32 +849,41 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0s,
                                 g.F == 1 ? species : nullptr, NW);
     // this wave's atoms in staging order: a ballot on its slab bit per 64
-    // records, then the set bits in turn; the next atom's record and kernel
-    // values are read while the current one is spread
-    for (int b0 = 0; b0 < nst; b0 += 64) {
-      const int idx = b0 + lane;
-      const int wv = idx < nst ? rec[idx].w : 0;
-      unsigned long long m = __ballot((wv >> (24 + wave)) & 1);
-      if (!m) continue;
-      int u = __ffsll((long long)m) - 1;
-      m &= m - 1;
-      int4 rc = rec[b0 + u];
-      T pc = lane < 3 * ns ? phi[(size_t)rc.x * 3 * ns + lane] : T(0);
-      while (u >= 0) {
-        const int un = m ? __ffsll((long long)m) - 1 : -1;
-        int4 rn = rc;
-        T pn = pc;
-        if (un >= 0) {
-          m &= m - 1;
-          rn = rec[b0 + un];
-          if (lane < 3 * ns) pn = phi[(size_t)rn.x * 3 * ns + lane];
+    // records, then the set bits in groups whose kernel values are loaded
+    // together (A atoms per register, at lanes 0 and 32 when A = 2)
+    auto walk = [&](auto per_reg) {
+      constexpr int A = decltype(per_reg)::value;
+      for (int b0 = 0; b0 < nst; b0 += 64) {
+        const int idx = b0 + lane;
+        const int wv = idx < nst ? rec[idx].w : 0;
+        unsigned long long m = __ballot((wv >> (24 + wave)) & 1);
+        while (m) {
+          int ub[A * kPhiRegs];
+#pragma unroll
+          for (int i = 0; i < A * kPhiRegs; ++i) {
+            ub[i] = m ? __ffsll((long long)m) - 1 : -1;
+            m &= m - 1;
+          }
+          T pb[kPhiRegs];
+#pragma unroll
+          for (int q = 0; q < kPhiRegs; ++q) {
+            const int u = (A == 2 && lane >= 32) ? ub[A * q + A - 1] : ub[A * q];
+            const int kv = A == 2 ? (lane & 31) : lane;
+            pb[q] = T(0);
+            if (u >= 0 && kv < 3 * ns) pb[q] = phi[(size_t)rec[b0 + u].x * 3 * ns + kv];
+          }
+#pragma unroll
+          for (int i = 0; i < A * kPhiRegs; ++i) {
+            if (ub[i] < 0) break;
+            spread_atom(rec[b0 + ub[i]], pb[i / A], (i % A) * 32);
+          }
         }
-        spread_atom(rc, pc);
-        u = un;
-        rc = rn;
-        pc = pn;
       }
-    }
+    };
+    if (3 * ns <= 32)
+      walk(std::integral_constant<int, 2>());
+    else
+      walk(std::integral_constant<int, 1>());
     __syncthreads();  // the next chunk overwrites rec from its start
     if (tid == 0) st.nstaged = 0;
     __syncthreads();
