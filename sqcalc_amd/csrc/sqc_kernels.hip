@@ -731,7 +731,7 @@ constexpr int kWholeRounds = 3;
 constexpr int kPhiRegs = 4;
 
 struct WholeMap {
-  int xy4[kWholeRounds];   // 4 * xy cell (bpermute address of fxy and cofs)
+  int xy[kWholeRounds];    // xy cell
   int fz4[kWholeRounds];   // 4 * (2 ns + z point) (bpermute address of fz in ph)
   int z2[kWholeRounds];    // 2 * z point (offset in the interleaved tile)
 };
@@ -743,31 +743,39 @@ __device__ __forceinline__ WholeMap whole_map(int ns, int lane) {
   for (int h = 0; h < kWholeRounds; ++h) {
     const int c = lane + 64 * h;
     const int xy = small_div(c, inv_ns), z = c - xy * ns;
-    m.xy4[h] = 4 * xy;
+    m.xy[h] = xy;
     m.fz4[h] = 4 * (2 * ns + z);
     m.z2[h] = 2 * z;
   }
   return m;
 }
 
-// Add a whole stencil starting at z point a into the tile: all reads are
-// issued before the writes (the cells of one atom are distinct).
+// Add a whole stencil (ox x oy columns from tile column (xll, yll), kernel
+// points kxl, kyl on, z points a .. a + ns - 1) into the tile.  Each lane
+// finds its cell's column and LDS offset arithmetically and picks its three
+// kernel values out of ph (lanes po ..) itself, so the LDS reads of a round
+// depend on nothing but the record.  The product is formed as
+// ((w fx) fy) fz, the order of the partial-stencil path.
 template <typename T>
-__device__ __forceinline__ void spread_whole(T *tflat, T ph, T fxy, int cofs, int cells, int a,
-                                             int po4, const WholeMap &m, int lane) {
-  T f[kWholeRounds], r[kWholeRounds];
-  int o[kWholeRounds];
+__device__ __forceinline__ void spread_whole(T *tflat, T ph, T w, int ox, int kxl, int xll,
+                                             int kyl, int yll, int ns, int ccx, int cs, int cells,
+                                             int a, int po, const WholeMap &m, int lane) {
+  const float inv_ox = __builtin_amdgcn_rcpf((float)ox);
+  const int kx4 = 4 * (po + kxl), ky4 = 4 * (po + ns + kyl), po4 = 4 * po, a2 = 2 * a;
 #pragma unroll
   for (int h = 0; h < kWholeRounds; ++h) {
     if (64 * h < cells) {
-      f[h] = bperm(m.xy4[h], fxy) * bperm(m.fz4[h] + po4, ph);
-      o[h] = bperm(m.xy4[h], cofs) + 2 * a + m.z2[h];
-      r[h] = lane + 64 * h < cells ? tflat[o[h]] : T(0);
+      const int xy = m.xy[h];
+      const int iy = small_div(xy, inv_ox), ix = xy - iy * ox;
+      const int xl = xll + ix, yl = yll + iy;
+      const int o = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1) + a2 + m.z2[h];
+      const bool on = lane + 64 * h < cells;
+      T r = T(0);
+      if (on) r = tflat[o];
+      const T f = w * bperm(kx4 + 4 * ix, ph) * bperm(ky4 + 4 * iy, ph) * bperm(m.fz4[h] + po4, ph);
+      if (on) tflat[o] = r + f;
     }
   }
-#pragma unroll
-  for (int h = 0; h < kWholeRounds; ++h)
-    if (64 * h < cells && lane + 64 * h < cells) tflat[o[h]] = r[h] + f[h];
 }
 
 // ---------------------------------------------------------------------------
@@ -817,6 +825,11 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
     // (ox <= 16 and oy <= 2 by the tile choice, so ox*oy <= 64)
     const T w = (g.F == 1) ? (T)g.amp[(r.w >> 16) & 0xFF] : T(1);
+    const int cells = ox * oy * ns;
+    if (a >= zs0 && a + ns <= zs1 && cells <= 64 * kWholeRounds) {
+      spread_whole<T>(tflat, ph, w, ox, kxl, xll, kyl, yll, ns, ccx, cs, cells, a, po, wm, lane);
+      return;
+    }
     T fxy;
     int cofs;
     {
@@ -827,11 +840,6 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
       fxy = w * __shfl(ph, po + kxl + ix) * __shfl(ph, po + ns + kyl + iy);
       const int xl = xll + ix, yl = yll + iy;
       cofs = 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
-    }
-    const int cells = ox * oy * ns;
-    if (a >= zs0 && a + ns <= zs1 && cells <= 64 * kWholeRounds) {
-      spread_whole<T>(tflat, ph, fxy, cofs, cells, a, 4 * po, wm, lane);
-      return;
     }
     // the stencil's z range inside this wave's slab: one piece, or two
     // when it wraps through z = 0
