@@ -1037,7 +1037,9 @@ __device__ __forceinline__ void load_group(LineGroup &lg, int l0, int nl,
   gl.put(lg, nl);
 }
 
-template <typename T, int N>
+// FC: the number of fields when the epilogue is compiled for it (0: g.F at
+// run time).  With more than one field, fields and species coincide.
+template <typename T, int N, int FC>
 __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, int ngroups,
                                                   const typename C2<T>::t *__restrict__ W2,
                                                   const typename C2<T>::t *__restrict__ twx,
@@ -1052,7 +1054,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LineGroup lg;
-  const int n1 = g.n[0], F = g.F, nq = g.nq, ncomp = g.ncomp, M1 = g.M[0];
+  const int n1 = g.n[0], F = FC > 0 ? FC : g.F, nq = g.nq, ncomp = g.ncomp, M1 = g.M[0];
+  constexpr int FM = FC > 0 ? FC : kMaxSpecies;
   const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
   V *tw = rows + (size_t)lb * F * rs;
@@ -1062,7 +1065,6 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
   for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
   for (int i = tid; i <= M1; i += NT) inv1[i] = 1.0 / ph1[i];
-  const int S = g.S;
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int l0 = grp * lb;
     const int nl = min(lb, g.nlines - l0);
@@ -1093,15 +1095,14 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
     // one line at a time: pairs raise the register count past three waves per SIMD
     for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
     __syncthreads();
-    auto bin_mode = [&](int c, unsigned s) {
-      if (s == kNoShell) return;
-      int li = 0;
-      while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
+    // candidate c of line li, kept in shell s
+    auto bin_mode = [&](int c, int li, unsigned s) {
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
       const int h1 = lg.h[li][2] + (c - lg.off[li]);
       const double d = lg.d23[li] * inv1[abs(h1)];
       const int ix = h1 < 0 ? h1 + n1 : h1;
-      double rr[kMaxSpecies], ri[kMaxSpecies];
+      double rr[FM], ri[FM];
+#pragma unroll
       for (int f = 0; f < F; ++f) {
         const V v = rows[((size_t)li * F + f) * rs + (ix)];
         rr[f] = (double)v.x * d;
@@ -1116,7 +1117,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
         const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
         tr = 0.0;
         ti = 0.0;
-        for (int a = 0; a < S; ++a) {
+#pragma unroll
+        for (int a = 0; a < F; ++a) {
           const double w = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
           tr += w * rr[a];
           ti += w * ri[a];
@@ -1127,19 +1129,25 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
       if (gnum) gnum[c] += val;
       if (g.partials && F > 1) {
         int pidx = 1;
-        for (int a = 0; a < S; ++a)
-          for (int b = a; b < S; ++b, ++pidx)
+#pragma unroll
+        for (int a = 0; a < F; ++a)
+#pragma unroll
+          for (int b = a; b < F; ++b, ++pidx)
             atomicAdd(&acc[pidx * nq + s], 2.0 * (rr[a] * rr[b] + ri[a] * ri[b]));
       }
     };
     // one copy of the body: the prefetched index of round k is picked by a
-    // compare-select chain, so sv stays in registers
+    // compare-select chain, so sv stays in registers.  A thread's candidates
+    // ascend, so its line search resumes where the previous one stopped.
+    int li = 0;
 #pragma unroll 1
     for (int k = 0, c = c0 + tid; c < c1; ++k, c += NT) {
       unsigned sh = sv[0];
 #pragma unroll
       for (int u = 1; u < kPreShell; ++u) sh = (k == u) ? sv[u] : sh;
-      bin_mode(c, k < kPreShell ? sh : (unsigned)stab[c]);
+      const unsigned s = k < kPreShell ? sh : (unsigned)stab[c];
+      while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
+      if (s != kNoShell) bin_mode(c, li, s);
     }
   }
   __syncthreads();
@@ -1622,6 +1630,14 @@ template <> struct FixedSizes<double> {
   }
 };
 
+// Field counts the x-pass epilogue is compiled for; others run its loops
+// with the count read at run time.
+template <typename Fn> static void pick_fields(int F, Fn &&fn) {
+  if (F == 1) return fn(std::integral_constant<int, 1>{});
+  if (F == 2) return fn(std::integral_constant<int, 2>{});
+  fn(std::integral_constant<int, 0>{});
+}
+
 template <typename T>
 static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
   Geo &g = p.geo;
@@ -1668,11 +1684,13 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
   {
     StageTimer t(p, 3);
     FixedSizes<T>::pick(g.n[0], [&](auto n) {
-      hipLaunchKernelGGL((k_fft_x_bin<T, decltype(n)::value>), dim3(p.nblk3), dim3(NT), p.lds3,
-                         st, g, p.ax[0], p.lb, ngroups, (const V *)p.d_W2, (const V *)p.d_tw[0],
-                         (const int4 *)p.d_line_info, p.d_line_off, p.d_stab, p.d_ph[0],
-                         p.d_line_d23, p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
-                         mode == 0 ? p.d_gnum : nullptr);
+      pick_fields(g.F, [&](auto fc) {
+        hipLaunchKernelGGL((k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>),
+                           dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
+                           (const V *)p.d_W2, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
+                           p.d_line_off, p.d_stab, p.d_ph[0], p.d_line_d23, p.d_wgpart,
+                           (V *)rho_out, (int64_t)p.ncand, mode == 0 ? p.d_gnum : nullptr);
+      });
     });
   }
   SQC_CHECK(hipGetLastError());
@@ -1772,7 +1790,9 @@ static int lds_limits_t(Plan &p) {
     set((const void *)k_adj_y<T, decltype(n)::value>, p.lds2);
   });
   FixedSizes<T>::pick(p.geo.n[0], [&](auto n) {
-    set((const void *)k_fft_x_bin<T, decltype(n)::value>, p.lds3);
+    pick_fields(p.geo.F, [&](auto fc) {
+      set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>, p.lds3);
+    });
     set((const void *)k_adj_x<T, decltype(n)::value>, p.lds3);
   });
   return err;
@@ -1786,9 +1806,12 @@ static void persistent_grids_t(Plan &p) {
   const Geo &g = p.geo;
   int nx = 0;
   FixedSizes<T>::pick(g.n[0], [&](auto n) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nx, k_fft_x_bin<T, decltype(n)::value>, NT,
-                                                     p.lds3) != hipSuccess)
-      nx = 0;
+    pick_fields(g.F, [&](auto fc) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nx, k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>, NT, p.lds3) !=
+          hipSuccess)
+        nx = 0;
+    });
   });
   (void)hipGetLastError();
   if (nx > 0) {
