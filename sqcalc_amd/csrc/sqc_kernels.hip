@@ -1210,8 +1210,8 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ wgp
 // Adjoint.  V_c(x) = sum_m C_cm exp(i h_m . x) is real; its gradient at the
 // atoms is interpolated with the kernel derivative.
 // ---------------------------------------------------------------------------
-template <typename T, int N>
-__global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int ngroups, int Fa,
+template <typename T, int N, int FC>
+__global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int ngroups, int Fa_,
                                               const typename C2<T>::t *__restrict__ rho,
                                               int64_t ncand,
                                               const typename C2<T>::t *__restrict__ twx,
@@ -1227,7 +1227,10 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
                                               typename C2<T>::t *__restrict__ W2) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n1 = g.n[0], F = g.F, nq = g.nq, S = g.S;
+  // fields, species and adjoint fields coincide when there is more than one
+  const int n1 = g.n[0], F = FC > 0 ? FC : g.F, nq = g.nq, Fa = FC > 0 ? FC : Fa_;
+  const int S = F > 1 ? F : g.S;
+  constexpr int FM = FC > 0 ? FC : kMaxSpecies;
   __shared__ LineGroup lg;
   const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
@@ -1257,17 +1260,16 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
         rv[k] = rho[c];
       }
     }
-    auto amplify = [&](int c, unsigned s, V r0) {
-      if (s == kNoShell) return;
-      int li = 0;
-      while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
+    // candidate c of line li, kept in shell s
+    auto amplify = [&](int c, int li, unsigned s, V r0) {
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
       const int h1 = lg.h[li][2] + (c - lg.off[li]);
       const double d = lg.d23[li] * inv1[abs(h1)];
       const int ix = h1 < 0 ? h1 + n1 : h1;
-      double rr[kMaxSpecies], ri[kMaxSpecies], w[kMaxSpecies];
+      double rr[FM], ri[FM], w[FM];
       rr[0] = r0.x;
       ri[0] = r0.y;
+#pragma unroll
       for (int f = 1; f < F; ++f) {
         const V v = rho[(size_t)f * ncand + c];
         rr[f] = v.x;
@@ -1279,20 +1281,25 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
         ti = ri[0];
       } else {
         const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
-        for (int a = 0; a < S; ++a) {
+#pragma unroll
+        for (int a = 0; a < F; ++a) {
           w[a] = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
           tr += w[a] * rr[a];
           ti += w[a] * ri[a];
         }
       }
       const double G = den[s] > 0.0 ? gS[s] / den[s] : 0.0;
+#pragma unroll
       for (int cf = 0; cf < Fa; ++cf) {
         const double wc = (Fa == 1) ? 1.0 : w[cf];
         double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
         if (gP) {
+          // pair gradients exist only with one field per species
           const double nc = (double)g.natoms * fmax(cnt[s], 1.0);
-          for (int a = 0; a < S; ++a)
-            for (int b = a; b < S; ++b) {
+#pragma unroll
+          for (int a = 0; a < F; ++a)
+#pragma unroll
+            for (int b = a; b < F; ++b) {
               const double H = gP[((size_t)a * S + b) * nq + s] / nc;
               if (cf == a) { cr += H * rr[b]; ci -= H * ri[b]; }
               if (cf == b) { cr += H * rr[a]; ci -= H * ri[a]; }
@@ -1301,6 +1308,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
         rows[((size_t)li * Fa + cf) * rs + (ix)] = V{(T)(cr * d), (T)(ci * d)};
       }
     };
+    int li = 0;  // a thread's candidates ascend: the line search resumes
 #pragma unroll 1
     for (int k = 0, c = c0 + tid; c < c1; ++k, c += NT) {
       unsigned sh = sv[0];
@@ -1314,7 +1322,8 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
         sh = stab[c];
         r0 = rho[c];
       }
-      amplify(c, sh, r0);
+      while (li + 1 < nl && lg.off[li + 1] <= c) ++li;
+      if (sh != kNoShell) amplify(c, li, sh, r0);
     }
     __syncthreads();
     for (int r = wave; r < nl * Fa; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
@@ -1727,11 +1736,13 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   {
     StageTimer t(p, 5);
     FixedSizes<T>::pick(g.n[0], [&](auto n) {
-      hipLaunchKernelGGL((k_adj_x<T, decltype(n)::value>), dim3(p.nblk3), dim3(NT), p.lds3, st, g,
-                         p.ax[0], p.lb, ngroups, Fa, (const V *)rho, (int64_t)p.ncand,
-                         (const V *)p.d_tw[0], (const int4 *)p.d_line_info, p.d_line_off,
-                         p.d_stab, p.d_ph[0], p.d_line_d23, p.d_den, p.d_cnt, gS, gP,
-                         (V *)p.d_W2);
+      pick_fields(g.F, [&](auto fc) {
+        hipLaunchKernelGGL((k_adj_x<T, decltype(n)::value, decltype(fc)::value>), dim3(p.nblk3),
+                           dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups, Fa, (const V *)rho,
+                           (int64_t)p.ncand, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
+                           p.d_line_off, p.d_stab, p.d_ph[0], p.d_line_d23, p.d_den, p.d_cnt,
+                           gS, gP, (V *)p.d_W2);
+      });
     });
   }
   {
@@ -1793,7 +1804,9 @@ static int lds_limits_t(Plan &p) {
     pick_fields(p.geo.F, [&](auto fc) {
       set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>, p.lds3);
     });
-    set((const void *)k_adj_x<T, decltype(n)::value>, p.lds3);
+    pick_fields(p.geo.F, [&](auto fc) {
+      set((const void *)k_adj_x<T, decltype(n)::value, decltype(fc)::value>, p.lds3);
+    });
   });
   return err;
 }
