@@ -3,7 +3,6 @@
 // Data layout in HBM (F = fields, i.e. species transforms or one weighted
 // transform; T2 = float2 or double2; n = fine grid, M = (m-1)/2):
 //   pos      [batch][natoms][3]       f64   Cartesian input, frame_t%pos order
-//   X        [natoms][3]              f64   grid coordinates s*n of the frame
 //   W1       [F][M3+1][n2][n1]        T2    after spread + z-DFT, h3 >= 0 only
 //   W2       [F][nlines][n1]          T2    after the y-DFT, canonical (h2,h3) lines
 //   stab     [ncand]                  u16   shell of each candidate mode (0xFFFF: not kept)
@@ -118,7 +117,6 @@ struct Plan {
   int use_graph = 1;
   // device buffers
   double *d_pos = nullptr;
-  double *d_X = nullptr;
   int4 *d_i0 = nullptr;        // stencil start per atom (i0x, i0y, i0z, -)
   int4 *d_i0s = nullptr;       // the same in sorted (bin) order
   void *d_phi = nullptr;       // kernel values per atom [3][ns], path precision

@@ -1,8 +1,8 @@
 // sqc_kernels.hip -- gfx950 kernels of the reciprocal-space S(q) path.
 //
 // Forward, per frame (one launch each):
-//   k_prep        wrap the atoms into the cell (FP64), grid coordinates, stencil
-//                 start and the 3 x ns kernel values of every atom, tile bin
+//   k_prep        wrap the atoms into the cell (FP64): stencil start, the
+//                 3 x ns kernel values and the tile bin of every atom
 //   k_scan        exclusive scan of the bin counts (one workgroup)
 //   k_scatter     counting-sort scatter of atom indices by bin
 //   k_binsort     order each bin by atom index (makes the spread deterministic)
@@ -411,99 +411,117 @@ __device__ __forceinline__ int small_div(int c, float inv_d) { return (int)(((fl
 // ---------------------------------------------------------------------------
 // Atom preparation and counting sort
 // ---------------------------------------------------------------------------
+// One thread per kernel value (atom j, axis i, point k): consecutive threads
+// write consecutive values of the per-atom [3][ns] table.  The thread of
+// point 0 of axis 0 also writes the atom's stencil corner and bin key.
 template <typename T>
 __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__restrict__ species,
-                       double *__restrict__ X, int4 *__restrict__ i0o, T *__restrict__ phi,
-                       T *__restrict__ dphi, int32_t *__restrict__ key,
-                       int32_t *__restrict__ count) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= g.natoms) return;
-  const double d0 = pos[3 * j] - g.origin[0];
-  const double d1 = pos[3 * j + 1] - g.origin[1];
-  const double d2 = pos[3 * j + 2] - g.origin[2];
-  double x[3];
-  int i0[3];
-  const int ns = g.ns;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
+                       int4 *__restrict__ i0o, T *__restrict__ phi, T *__restrict__ dphi,
+                       int32_t *__restrict__ key, int32_t *__restrict__ count) {
+  const int ns = g.ns, per = 3 * ns;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;  // < natoms * 3 ns < 2^31
+  const unsigned j = t / (unsigned)per;
+  if (j >= (unsigned)g.natoms) return;
+  const int v = (int)(t - j * per), ia = v / ns, k = v - ia * ns;
+  const double d0 = pos[3 * (size_t)j] - g.origin[0];
+  const double d1 = pos[3 * (size_t)j + 1] - g.origin[1];
+  const double d2 = pos[3 * (size_t)j + 2] - g.origin[2];
+  // fractional coordinate, wrapped, and grid coordinate of axis i
+  auto grid_coord = [&](int i) {
     double s = g.ainv[i] * d0 + g.ainv[i + 3] * d1 + g.ainv[i + 6] * d2;
     s = s - floor(s);
     double u = s * (double)g.n[i];
     if (u >= (double)g.n[i]) u -= (double)g.n[i];
-    x[i] = u;
-    X[3 * j + i] = u;
-    i0[i] = (int)ceil(u - g.half_ns);
-    T *ph = phi + (size_t)j * 3 * ns + i * ns;
-    T *dp = dphi ? dphi + (size_t)j * 3 * ns + i * ns : nullptr;
-    for (int k = 0; k < ns; ++k) {
-      T d;
-      ph[k] = es_phi<T>((double)(i0[i] + k) - u, (double)ns, g.beta, dp ? &d : (T *)nullptr);
-      if (dp) dp[k] = d;
+    return u;
+  };
+  const double u = grid_coord(ia);
+  const int i0 = (int)ceil(u - g.half_ns);
+  T d;
+  phi[t] = es_phi<T>((double)(i0 + k) - u, (double)ns, g.beta, dphi ? &d : (T *)nullptr);
+  if (dphi) dphi[t] = d;
+  if (v == 0) {
+    double x[3];
+    int c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      x[i] = grid_coord(i);
+      c[i] = (int)ceil(x[i] - g.half_ns);
     }
+    i0o[j] = int4{c[0], c[1], c[2], 0};
+    const int tx = min((int)x[0] / g.txr, g.ntx - 1);
+    const int ty = min((int)x[1] / g.ty, g.nty - 1);
+    const int f = (g.F == 1) ? 0 : species[j];
+    const int b = (f * g.nty + ty) * g.ntx + tx;
+    key[j] = b;
+    atomicAdd(&count[b], 1);
   }
-  i0o[j] = int4{i0[0], i0[1], i0[2], 0};
-  const int tx = min((int)x[0] / g.txr, g.ntx - 1);
-  const int ty = min((int)x[1] / g.ty, g.nty - 1);
-  const int f = (g.F == 1) ? 0 : species[j];
-  const int k = (f * g.nty + ty) * g.ntx + tx;
-  key[j] = k;
-  atomicAdd(&count[k], 1);
 }
 
-// Exclusive scan of the bin counts by one workgroup: thread t owns the
-// contiguous chunk [t c, (t+1) c) with c a multiple of 4, read as int4 (all of
-// a thread's loads are independent, so they are in flight together); the
-// chunk sums are scanned across the block, then every bin gets its start, the
-// scatter cursor, and its count cleared for the next frame.
+// Exclusive scan of the nb bin counts by one workgroup: each wave owns a
+// contiguous run of 4-count groups, read coalesced (one group per lane per
+// round); wave totals are combined through LDS, then every wave scans its
+// run with lane shuffles.  The counts are cleared for the next frame.
 __global__ void __launch_bounds__(1024) k_scan(int32_t *__restrict__ count, int32_t *__restrict__ start,
                                                int32_t *__restrict__ cursor, int nb) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  const int chunk = (((nb + 1023) / 1024) + 3) & ~3;
-  const int b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
-  int s = 0;
-  {
-    const int nv = (b1 - b0) >> 2;
-    const int4 *c4 = reinterpret_cast<const int4 *>(count + b0);
-#pragma unroll 8
-    for (int v = 0; v < nv; ++v) {
-      const int4 c = c4[v];
-      s += c.x + c.y + c.z + c.w;
-    }
-    for (int b = b0 + 4 * nv; b < b1; ++b) s += count[b];
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ng = (nb + 3) >> 2;
+  const int seg = (ng + 15) >> 4;
+  const int g0 = min(ng, wave * seg), g1 = min(ng, g0 + seg);
+  auto load4 = [&](int gi) {
+    if (4 * gi + 3 < nb) return reinterpret_cast<const int4 *>(count)[gi];
+    int4 c{0, 0, 0, 0};
+    const int b = 4 * gi;
+    if (b < nb) c.x = count[b];
+    if (b + 1 < nb) c.y = count[b + 1];
+    if (b + 2 < nb) c.z = count[b + 2];
+    return c;
+  };
+  int acc = 0;
+  for (int gi = g0 + lane; gi < g1; gi += 64) {
+    const int4 c = load4(gi);
+    acc += c.x + c.y + c.z + c.w;
   }
-  part[tid] = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) wsum[wave] = acc;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  int carry = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) {
+    carry += w < wave ? wsum[w] : 0;
+    total += wsum[w];
   }
-  int run = tid ? part[tid - 1] : 0;
-  {
-    const int nv = (b1 - b0) >> 2;
-    int4 *c4 = reinterpret_cast<int4 *>(count + b0);
-    int4 *s4 = reinterpret_cast<int4 *>(start + b0);
-    int4 *u4 = reinterpret_cast<int4 *>(cursor + b0);
-#pragma unroll 4
-    for (int v = 0; v < nv; ++v) {
-      const int4 c = c4[v];
-      const int4 o = int4{run, run + c.x, run + c.x + c.y, run + c.x + c.y + c.z};
-      s4[v] = o;
-      u4[v] = o;
-      c4[v] = int4{0, 0, 0, 0};  // ready for the next frame
-      run = o.w + c.w;
+  if (tid == 0) start[nb] = total;
+  for (int base = g0; base < g1; base += 64) {
+    const int gi = base + lane;
+    const int4 c = gi < g1 ? load4(gi) : int4{0, 0, 0, 0};
+    const int sum4 = c.x + c.y + c.z + c.w;
+    int incl = sum4;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
     }
-    for (int b = b0 + 4 * nv; b < b1; ++b) {
-      const int c = count[b];
-      start[b] = run;
-      cursor[b] = run;
-      count[b] = 0;
-      run += c;
+    const int o = carry + incl - sum4;
+    const int4 r{o, o + c.x, o + c.x + c.y, o + c.x + c.y + c.z};
+    if (gi < g1) {
+      const int b = 4 * gi;
+      if (b + 3 < nb) {
+        reinterpret_cast<int4 *>(start)[gi] = r;
+        reinterpret_cast<int4 *>(cursor)[gi] = r;
+        reinterpret_cast<int4 *>(count)[gi] = int4{0, 0, 0, 0};  // ready for the next frame
+      } else {
+        const int rv[3] = {r.x, r.y, r.z};
+        for (int e = 0; e < 3 && b + e < nb; ++e) {
+          start[b + e] = rv[e];
+          cursor[b + e] = rv[e];
+          count[b + e] = 0;
+        }
+      }
     }
+    carry += __shfl(incl, 63);
   }
-  if (tid == 1023) start[nb] = part[1023];
 }
 
 __global__ void k_scatter(int64_t natoms, const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
@@ -1655,8 +1673,9 @@ static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
   const int blk = 256;
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
   StageTimer t(p, 0);
-  hipLaunchKernelGGL(k_prep<T>, dim3(ga), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_X,
-                     p.d_i0, (T *)p.d_phi, with_dphi ? (T *)p.d_dphi : (T *)nullptr, p.d_key,
+  const unsigned gv = (unsigned)((g.natoms * 3 * g.ns + blk - 1) / blk);
+  hipLaunchKernelGGL(k_prep<T>, dim3(gv), dim3(blk), 0, st, g, d_pos, p.d_species, p.d_i0,
+                     (T *)p.d_phi, with_dphi ? (T *)p.d_dphi : (T *)nullptr, p.d_key,
                      p.d_count);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, p.d_count, p.d_start, p.d_cursor,
                      nbins_tot);
