@@ -256,9 +256,10 @@ double amplitude(const sqc::Geo &g, int sp, double q) {
   return g.amp[sp];
 }
 
-// Spread tiles (txr real x-columns, ty y-columns), largest first.  txr <= 16
-// and ty <= 2 keep a stencil's xy footprint inside a tile at 64 cells or
-// fewer, which the spreader's per-lane xy table relies on.
+// Spread tiles (txr real x-columns, ty y-columns), largest first.  Every
+// tile has txr * ty <= 32, which keeps a stencil's xy footprint inside a
+// tile at 64 cells or fewer (the spreader's per-lane xy table relies on it),
+// and txr even (two real columns per complex DFT line).
 constexpr int kTiles[][2] = {{16, 2}, {8, 2}, {16, 1}, {4, 2}, {8, 1}, {4, 1}, {2, 1}};
 constexpr int kNumTiles = 7;
 constexpr size_t kLdsCap = 160 * 1024;
@@ -820,10 +821,12 @@ int tune(sqc::Plan &p, const double *pos_dev) {
   int btxr = p.geo.txr, bty = p.geo.ty, btx2 = p.tx2, blb = p.lb;
   double st[4];
   if (int rc = time_stages(p, pos_dev, st)) return rc;
-  double best = st[0] + st[1];
-  // spread tiles: the best and the runner-up of a first sweep are timed
-  // again, in turn, before one is kept (neighbouring tiles often differ by
-  // a few percent, within the spread of a single timing)
+  // spread tiles, judged on the sort, the spread and the y pass (which reads
+  // the tile's write pattern): the best and the runner-up of a first sweep
+  // are timed again, in turn, before one is kept (neighbouring tiles often
+  // differ by a few percent, within the spread of a single timing)
+  auto tile_cost = [&]() { return st[0] + st[1] + st[2]; };
+  double best = tile_cost();
   int rtxr = 0, rty = 0;
   double second = 1e30;
   for (int c = 0; c < kNumTiles; ++c) {
@@ -831,7 +834,7 @@ int tune(sqc::Plan &p, const double *pos_dev) {
     if ((txr == btxr && ty == bty) || !tile_fits(p, txr, ty, kLdsCap)) continue;
     if (int rc = apply_tiles(p, txr, ty, btx2, blb)) return rc;
     if (int rc = time_stages(p, pos_dev, st)) return rc;
-    const double v = st[0] + st[1];
+    const double v = tile_cost();
     if (v < kGain * best) {
       second = best;
       rtxr = btxr;
@@ -850,10 +853,10 @@ int tune(sqc::Plan &p, const double *pos_dev) {
     for (int rep = 0; rep < 2; ++rep) {
       if (int rc = apply_tiles(p, rtxr, rty, btx2, blb)) return rc;
       if (int rc = time_stages(p, pos_dev, st)) return rc;
-      tr = std::min(tr, st[0] + st[1]);
+      tr = std::min(tr, tile_cost());
       if (int rc = apply_tiles(p, btxr, bty, btx2, blb)) return rc;
       if (int rc = time_stages(p, pos_dev, st)) return rc;
-      tb = std::min(tb, st[0] + st[1]);
+      tb = std::min(tb, tile_cost());
     }
     if (tr < kGain * tb) {
       btxr = rtxr;

@@ -841,7 +841,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
     const int a = r.w & 0xFFFF;
     // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
-    // (ox <= 16 and oy <= 2 by the tile choice, so ox*oy <= 64)
+    // (ox <= txr and oy <= ty with txr * ty <= 32, so ox*oy <= 64)
     const T w = (g.F == 1) ? (T)g.amp[(r.w >> 16) & 0xFF] : T(1);
     const int cells = ox * oy * ns;
     if (a >= zs0 && a + ns <= zs1 && cells <= 64 * kWholeRounds) {
