@@ -402,7 +402,7 @@ void drop_graph(sqc::Plan &p) {
 void free_plan(sqc_hip_plan *h) {
   sqc::Plan &p = h->p;
   void *bufs[] = {p.d_pos, p.d_i0, p.d_i0s, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
-                  p.d_species, p.d_W1, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
+                  p.d_species, p.d_W1, p.d_W1b, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
                   p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
                   p.d_line_off, p.d_line_d23, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
                   p.d_gnum, p.d_slots, p.d_gs, p.d_rho, h->g_pos, h->g_out, h->g_gs,
@@ -416,6 +416,8 @@ void free_plan(sqc_hip_plan *h) {
   }
   for (hipEvent_t e : p.ev_free) (void)hipEventDestroy(e);
   drop_graph(p);
+  for (hipEvent_t e : p.gev) (void)hipEventDestroy(e);
+  if (p.stream2) (void)hipStreamDestroy(p.stream2);
   if (p.ev_in) (void)hipEventDestroy(p.ev_in);
   if (p.ev_out) (void)hipEventDestroy(p.ev_out);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
@@ -689,6 +691,11 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   const size_t w1 = (size_t)g.F * (M3 + 1) * n2 * n1 * vs;
   const size_t w2 = (size_t)g.F * std::max(g.nlines, 1) * n1 * vs;
   if ((rc = alloc((char **)&p.d_W1, w1, p.bytes))) return rc;
+  if (p.use_graph) {
+    // captured batches overlap consecutive frames: a second W1 and stream
+    if ((rc = alloc((char **)&p.d_W1b, w1, p.bytes))) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&p.stream2, hipStreamNonBlocking), "hipStreamCreate");
+  }
   if ((rc = alloc((char **)&p.d_W2, w2, p.bytes))) return rc;
   for (int i = 0; i < 3; ++i)
     if ((rc = upload_twiddles(p, i))) return rc;
@@ -890,14 +897,38 @@ int tune(sqc::Plan &p, const double *pos_dev) {
   return apply_tiles(p, btxr, bty, btx2, blb);
 }
 
-// Capture batch_cap forward frames reading p.d_pos into one graph.
+// Capture batch_cap forward frames reading p.d_pos into one graph.  Frame
+// f's atom sort and spread (stream) write W1 or W1b by the parity of f; its
+// y pass, x pass and finalize (stream2) follow them.  So the sort and spread
+// of frame f + 1 run beside the y and x passes of frame f; the spread of
+// frame f + 2 waits for the passes of frame f, which read the same W1.  The
+// finalizes keep their order, so the sums are those of frames launched one
+// by one.
 int capture_batch(sqc::Plan &p) {
   const int64_t stride = 3 * p.geo.natoms;
+  const size_t nev = 2 * (size_t)p.batch_cap + 1;
+  while (p.gev.size() < nev) {
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    p.gev.push_back(e);
+  }
+  hipEvent_t *evA = p.gev.data() + 1, *evB = p.gev.data() + 1 + p.batch_cap;
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-  int rc = 0;
-  for (int64_t f = 0; f < p.batch_cap && !rc; ++f)
-    rc = sqc::launch_forward(p, p.d_pos + f * stride, 0, nullptr, nullptr);
+  hipError_t ec = hipEventRecord(p.gev[0], p.stream);
+  if (ec == hipSuccess) ec = hipStreamWaitEvent(p.stream2, p.gev[0], 0);
+  int rc = ec == hipSuccess ? 0 : (int)ec;
+  for (int64_t f = 0; f < p.batch_cap && !rc; ++f) {
+    void *W = (f & 1) ? p.d_W1b : p.d_W1;
+    if (f >= 2 && (ec = hipStreamWaitEvent(p.stream, evB[f - 2], 0)) != hipSuccess) rc = (int)ec;
+    if (!rc) rc = sqc::launch_forward_front(p, p.d_pos + f * stride, W, p.stream);
+    if (!rc && (ec = hipEventRecord(evA[f], p.stream)) != hipSuccess) rc = (int)ec;
+    if (!rc && (ec = hipStreamWaitEvent(p.stream2, evA[f], 0)) != hipSuccess) rc = (int)ec;
+    if (!rc) rc = sqc::launch_forward_back(p, W, p.stream2, 0, nullptr, nullptr);
+    if (!rc && (ec = hipEventRecord(evB[f], p.stream2)) != hipSuccess) rc = (int)ec;
+  }
+  if (!rc && (ec = hipStreamWaitEvent(p.stream, evB[p.batch_cap - 1], 0)) != hipSuccess)
+    rc = (int)ec;
   const hipError_t e = hipStreamEndCapture(p.stream, &graph);
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);

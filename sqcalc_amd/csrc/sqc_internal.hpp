@@ -115,6 +115,11 @@ struct Plan {
   // batched frames: a captured graph of batch_cap frames read from d_pos
   hipGraphExec_t gexec = nullptr;
   int use_graph = 1;
+  // the graph runs the y and x passes of frame i on stream2 while the atom
+  // sort and spread of frame i + 1 run on stream, W1 alternating with W1b
+  hipStream_t stream2 = nullptr;
+  void *d_W1b = nullptr;
+  std::vector<hipEvent_t> gev;   // fork/join events of the capture
   // device buffers
   double *d_pos = nullptr;
   int4 *d_i0 = nullptr;        // stencil start per atom (i0x, i0y, i0z, -)
@@ -153,7 +158,10 @@ int launch_forward(Plan &p, const double *d_pos_frame, int mode, double *frame_o
                    void *rho_out);
 int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const double *gS,
                    const double *gP, double *grad);
-int launch_finalize(Plan &p, double *frame_out, int accumulate);
+int launch_forward_front(Plan &p, const double *d_pos_frame, void *W1, hipStream_t st);
+int launch_forward_back(Plan &p, const void *W1, hipStream_t st, int mode, double *frame_out,
+                        void *rho_out);
+int launch_finalize(Plan &p, double *frame_out, int accumulate, hipStream_t st);
 int kernel_lds_limits(Plan &p);
 void persistent_grids(Plan &p);
 int collect_stage_times(Plan &p);

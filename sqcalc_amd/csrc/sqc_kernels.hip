@@ -1666,9 +1666,8 @@ template <typename Fn> static void pick_fields(int F, Fn &&fn) {
 }
 
 template <typename T>
-static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
+static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi, hipStream_t st) {
   Geo &g = p.geo;
-  hipStream_t st = p.stream;
   const int nbins_tot = g.F * g.nbins;
   const int blk = 256;
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
@@ -1685,26 +1684,38 @@ static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi) {
                      p.d_sidx, p.d_i0, p.d_i0s);
 }
 
+// The forward pass in two halves: the atom sort and the spread write W1;
+// the y and x passes and the finalize read it.  A captured batch runs the
+// first half of frame i + 1 beside the second half of frame i (on two
+// streams, with W1 double-buffered).
 template <typename T>
-static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
+static int forward_front_t(Plan &p, const double *d_pos, void *W1, hipStream_t st) {
   using V = typename C2<T>::t;
   Geo &g = p.geo;
-  hipStream_t st = p.stream;
-  sort_atoms<T>(p, d_pos, false);
+  sort_atoms<T>(p, d_pos, false, st);
   {
     StageTimer t(p, 1);
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
       hipLaunchKernelGGL((k_spread_z<T, decltype(n)::value>), dim3(g.ntx * g.nty, g.F), dim3(NT),
                          p.lds1, st, g, p.ax[2], p.d_i0s, (const T *)p.d_phi, p.d_sidx, p.d_start,
-                         p.d_species, (const V *)p.d_tw[2], (V *)p.d_W1);
+                         p.d_species, (const V *)p.d_tw[2], (V *)W1);
     });
   }
+  SQC_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int forward_back_t(Plan &p, const void *W1, hipStream_t st, int mode, double *frame_out,
+                          void *rho_out) {
+  using V = typename C2<T>::t;
+  Geo &g = p.geo;
   {
     StageTimer t(p, 2);
     FixedSizes<T>::pick(g.n[1], [&](auto n) {
       hipLaunchKernelGGL((k_fft_y<T, decltype(n)::value>),
                          dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * g.F), dim3(NT), p.lds2, st,
-                         g, p.ax[1], p.tx2, (const V *)p.d_W1, (const V *)p.d_tw[1], p.d_lo2,
+                         g, p.ax[1], p.tx2, (const V *)W1, (const V *)p.d_tw[1], p.d_lo2,
                          p.d_hi2, p.d_loff, (V *)p.d_W2);
     });
   }
@@ -1722,22 +1733,33 @@ static int forward_t(Plan &p, const double *d_pos, int mode, double *frame_out, 
     });
   }
   SQC_CHECK(hipGetLastError());
-  return launch_finalize(p, frame_out, mode == 0);
+  return launch_finalize(p, frame_out, mode == 0, st);
 }
 
-int launch_finalize(Plan &p, double *frame_out, int accumulate) {
+int launch_finalize(Plan &p, double *frame_out, int accumulate, hipStream_t st) {
   const int ntot = p.geo.ncomp * p.geo.nq;
   StageTimer t(p, 4);
   hipLaunchKernelGGL(k_finalize, dim3((ntot + kFinEntries - 1) / kFinEntries), dim3(256), 0,
-                     p.stream, p.d_wgpart,
+                     st, p.d_wgpart,
                      p.nblk3, ntot, p.d_acc, frame_out, accumulate);
   SQC_CHECK(hipGetLastError());
   return 0;
 }
 
+int launch_forward_front(Plan &p, const double *d_pos, void *W1, hipStream_t st) {
+  return p.prec == 1 ? forward_front_t<float>(p, d_pos, W1, st)
+                     : forward_front_t<double>(p, d_pos, W1, st);
+}
+
+int launch_forward_back(Plan &p, const void *W1, hipStream_t st, int mode, double *frame_out,
+                        void *rho_out) {
+  return p.prec == 1 ? forward_back_t<float>(p, W1, st, mode, frame_out, rho_out)
+                     : forward_back_t<double>(p, W1, st, mode, frame_out, rho_out);
+}
+
 int launch_forward(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
-  return p.prec == 1 ? forward_t<float>(p, d_pos, mode, frame_out, rho_out)
-                     : forward_t<double>(p, d_pos, mode, frame_out, rho_out);
+  if (int rc = launch_forward_front(p, d_pos, p.d_W1, p.stream)) return rc;
+  return launch_forward_back(p, p.d_W1, p.stream, mode, frame_out, rho_out);
 }
 
 template <typename T>
@@ -1750,7 +1772,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   const unsigned ga = (unsigned)((g.natoms + blk - 1) / blk);
   const int Fa = g.F;
   // the atoms of this frame, sorted exactly as in the forward pass
-  sort_atoms<T>(p, d_pos, true);
+  sort_atoms<T>(p, d_pos, true, st);
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
   {
     StageTimer t(p, 5);
