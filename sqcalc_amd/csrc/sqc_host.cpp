@@ -691,12 +691,12 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   const size_t w1 = (size_t)g.F * (M3 + 1) * n2 * n1 * vs;
   const size_t w2 = (size_t)g.F * std::max(g.nlines, 1) * n1 * vs;
   if ((rc = alloc((char **)&p.d_W1, w1, p.bytes))) return rc;
+  if ((rc = alloc((char **)&p.d_W2, w2, p.bytes))) return rc;
   if (p.use_graph) {
     // captured batches overlap consecutive frames: a second W1 and stream
     if ((rc = alloc((char **)&p.d_W1b, w1, p.bytes))) return rc;
     HIP_TRY(hipStreamCreateWithFlags(&p.stream2, hipStreamNonBlocking), "hipStreamCreate");
   }
-  if ((rc = alloc((char **)&p.d_W2, w2, p.bytes))) return rc;
   for (int i = 0; i < 3; ++i)
     if ((rc = upload_twiddles(p, i))) return rc;
   std::vector<double> phv[3];
@@ -897,38 +897,46 @@ int tune(sqc::Plan &p, const double *pos_dev) {
   return apply_tiles(p, btxr, bty, btx2, blb);
 }
 
-// Capture batch_cap forward frames reading p.d_pos into one graph.  Frame
-// f's atom sort and spread (stream) write W1 or W1b by the parity of f; its
-// y pass, x pass and finalize (stream2) follow them.  So the sort and spread
-// of frame f + 1 run beside the y and x passes of frame f; the spread of
-// frame f + 2 waits for the passes of frame f, which read the same W1.  The
-// finalizes keep their order, so the sums are those of frames launched one
-// by one.
+// Capture batch_cap forward frames reading p.d_pos into one graph, as a
+// two-stage pipeline: frame f's atom sort and spread (stream) write W1 or
+// W1b by the parity of f; its y pass, x pass and finalize (stream2) follow
+// them.  So the sort and spread of frame f + 1 run beside the y and x passes
+// of frame f; the spread of frame f + 2 waits for the y pass of frame f,
+// which reads the same W1.  The finalizes keep their order, so the sums are
+// those of frames launched one by one.
 int capture_batch(sqc::Plan &p) {
   const int64_t stride = 3 * p.geo.natoms;
-  const size_t nev = 2 * (size_t)p.batch_cap + 1;
+  const int64_t B = p.batch_cap;
+  const size_t nev = 2 * (size_t)B + 1;
   while (p.gev.size() < nev) {
     hipEvent_t e = nullptr;
     HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     p.gev.push_back(e);
   }
-  hipEvent_t *evA = p.gev.data() + 1, *evB = p.gev.data() + 1 + p.batch_cap;
+  hipEvent_t *evS = p.gev.data() + 1, *evB = evS + B;
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-  hipError_t ec = hipEventRecord(p.gev[0], p.stream);
-  if (ec == hipSuccess) ec = hipStreamWaitEvent(p.stream2, p.gev[0], 0);
-  int rc = ec == hipSuccess ? 0 : (int)ec;
-  for (int64_t f = 0; f < p.batch_cap && !rc; ++f) {
-    void *W = (f & 1) ? p.d_W1b : p.d_W1;
-    if (f >= 2 && (ec = hipStreamWaitEvent(p.stream, evB[f - 2], 0)) != hipSuccess) rc = (int)ec;
-    if (!rc) rc = sqc::launch_forward_front(p, p.d_pos + f * stride, W, p.stream);
-    if (!rc && (ec = hipEventRecord(evA[f], p.stream)) != hipSuccess) rc = (int)ec;
-    if (!rc && (ec = hipStreamWaitEvent(p.stream2, evA[f], 0)) != hipSuccess) rc = (int)ec;
-    if (!rc) rc = sqc::launch_forward_back(p, W, p.stream2, 0, nullptr, nullptr);
-    if (!rc && (ec = hipEventRecord(evB[f], p.stream2)) != hipSuccess) rc = (int)ec;
+  int rc = 0;
+  // run one HIP call unless an earlier one failed
+  auto step = [&](auto &&call) {
+    if (rc) return;
+    const hipError_t e = call();
+    if (e != hipSuccess) rc = (int)e;
+  };
+  step([&] { return hipEventRecord(p.gev[0], p.stream); });
+  step([&] { return hipStreamWaitEvent(p.stream2, p.gev[0], 0); });
+  for (int64_t f = 0; f < B && !rc; ++f) {
+    void *W1 = (f & 1) ? p.d_W1b : p.d_W1;
+    if (f >= 2) step([&] { return hipStreamWaitEvent(p.stream, evB[f - 2], 0); });  // W1 reuse
+    if (!rc) rc = sqc::launch_forward_front(p, p.d_pos + f * stride, W1, p.stream);
+    step([&] { return hipEventRecord(evS[f], p.stream); });
+    step([&] { return hipStreamWaitEvent(p.stream2, evS[f], 0); });
+    if (!rc) rc = sqc::launch_forward_y(p, W1, p.d_W2, p.stream2);
+    if (!rc) rc = sqc::launch_forward_x(p, p.d_W2, p.stream2, 0, nullptr, nullptr);
+    step([&] { return hipEventRecord(evB[f], p.stream2); });
   }
-  if (!rc && (ec = hipStreamWaitEvent(p.stream, evB[p.batch_cap - 1], 0)) != hipSuccess)
-    rc = (int)ec;
+  // join the side stream back into the capturing stream
+  step([&] { return hipStreamWaitEvent(p.stream, evB[B - 1], 0); });
   const hipError_t e = hipStreamEndCapture(p.stream, &graph);
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);

@@ -159,8 +159,9 @@ int launch_forward(Plan &p, const double *d_pos_frame, int mode, double *frame_o
 int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const double *gS,
                    const double *gP, double *grad);
 int launch_forward_front(Plan &p, const double *d_pos_frame, void *W1, hipStream_t st);
-int launch_forward_back(Plan &p, const void *W1, hipStream_t st, int mode, double *frame_out,
-                        void *rho_out);
+int launch_forward_y(Plan &p, const void *W1, void *W2, hipStream_t st);
+int launch_forward_x(Plan &p, const void *W2, hipStream_t st, int mode, double *frame_out,
+                     void *rho_out);
 int launch_finalize(Plan &p, double *frame_out, int accumulate, hipStream_t st);
 int kernel_lds_limits(Plan &p);
 void persistent_grids(Plan &p);

@@ -1121,7 +1121,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
       const int ix = h1 < 0 ? h1 + n1 : h1;
       double rr[FM], ri[FM];
 #pragma unroll
-      for (int f = 0; f < F; ++f) {
+      for (int f = 0; f < FM && f < F; ++f) {
         const V v = rows[((size_t)li * F + f) * rs + (ix)];
         rr[f] = (double)v.x * d;
         ri[f] = (double)v.y * d;
@@ -1136,7 +1136,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
         tr = 0.0;
         ti = 0.0;
 #pragma unroll
-        for (int a = 0; a < F; ++a) {
+        for (int a = 0; a < FM && a < F; ++a) {
           const double w = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
           tr += w * rr[a];
           ti += w * ri[a];
@@ -1148,9 +1148,9 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
       if (g.partials && F > 1) {
         int pidx = 1;
 #pragma unroll
-        for (int a = 0; a < F; ++a)
+        for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
-          for (int b = a; b < F; ++b, ++pidx)
+          for (int b = a; b < FM && b < F; ++b, ++pidx)
             atomicAdd(&acc[pidx * nq + s], 2.0 * (rr[a] * rr[b] + ri[a] * ri[b]));
       }
     };
@@ -1288,7 +1288,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       rr[0] = r0.x;
       ri[0] = r0.y;
 #pragma unroll
-      for (int f = 1; f < F; ++f) {
+      for (int f = 1; f < FM && f < F; ++f) {
         const V v = rho[(size_t)f * ncand + c];
         rr[f] = v.x;
         ri[f] = v.y;
@@ -1300,7 +1300,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       } else {
         const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
 #pragma unroll
-        for (int a = 0; a < F; ++a) {
+        for (int a = 0; a < FM && a < F; ++a) {
           w[a] = (g.weight_kind == 2) ? xray_f(g.xcoef[a], q) : g.amp[a];
           tr += w[a] * rr[a];
           ti += w[a] * ri[a];
@@ -1308,16 +1308,16 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       }
       const double G = den[s] > 0.0 ? gS[s] / den[s] : 0.0;
 #pragma unroll
-      for (int cf = 0; cf < Fa; ++cf) {
+      for (int cf = 0; cf < FM && cf < Fa; ++cf) {
         const double wc = (Fa == 1) ? 1.0 : w[cf];
         double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
         if (gP) {
           // pair gradients exist only with one field per species
           const double nc = (double)g.natoms * fmax(cnt[s], 1.0);
 #pragma unroll
-          for (int a = 0; a < F; ++a)
+          for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
-            for (int b = a; b < F; ++b) {
+            for (int b = a; b < FM && b < F; ++b) {
               const double H = gP[((size_t)a * S + b) * nq + s] / nc;
               if (cf == a) { cr += H * rr[b]; ci -= H * ri[b]; }
               if (cf == b) { cr += H * rr[a]; ci -= H * ri[a]; }
@@ -1684,10 +1684,10 @@ static void sort_atoms(Plan &p, const double *d_pos, bool with_dphi, hipStream_t
                      p.d_sidx, p.d_i0, p.d_i0s);
 }
 
-// The forward pass in two halves: the atom sort and the spread write W1;
-// the y and x passes and the finalize read it.  A captured batch runs the
-// first half of frame i + 1 beside the second half of frame i (on two
-// streams, with W1 double-buffered).
+// The forward pass in three parts: the atom sort and the spread write W1;
+// the y pass reads W1 and writes W2; the x pass and the finalize read W2.  A
+// captured batch runs the sort and spread of frame i + 1 beside the y and x
+// passes of frame i (two streams, W1 double-buffered).
 template <typename T>
 static int forward_front_t(Plan &p, const double *d_pos, void *W1, hipStream_t st) {
   using V = typename C2<T>::t;
@@ -1706,8 +1706,7 @@ static int forward_front_t(Plan &p, const double *d_pos, void *W1, hipStream_t s
 }
 
 template <typename T>
-static int forward_back_t(Plan &p, const void *W1, hipStream_t st, int mode, double *frame_out,
-                          void *rho_out) {
+static int forward_y_t(Plan &p, const void *W1, void *W2, hipStream_t st) {
   using V = typename C2<T>::t;
   Geo &g = p.geo;
   {
@@ -1716,9 +1715,18 @@ static int forward_back_t(Plan &p, const void *W1, hipStream_t st, int mode, dou
       hipLaunchKernelGGL((k_fft_y<T, decltype(n)::value>),
                          dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * g.F), dim3(NT), p.lds2, st,
                          g, p.ax[1], p.tx2, (const V *)W1, (const V *)p.d_tw[1], p.d_lo2,
-                         p.d_hi2, p.d_loff, (V *)p.d_W2);
+                         p.d_hi2, p.d_loff, (V *)W2);
     });
   }
+  SQC_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int forward_x_t(Plan &p, const void *W2, hipStream_t st, int mode, double *frame_out,
+                       void *rho_out) {
+  using V = typename C2<T>::t;
+  Geo &g = p.geo;
   const int ngroups = (g.nlines + p.lb - 1) / p.lb;
   {
     StageTimer t(p, 3);
@@ -1726,7 +1734,7 @@ static int forward_back_t(Plan &p, const void *W1, hipStream_t st, int mode, dou
       pick_fields(g.F, [&](auto fc) {
         hipLaunchKernelGGL((k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>),
                            dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
-                           (const V *)p.d_W2, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
+                           (const V *)W2, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
                            p.d_line_off, p.d_stab, p.d_ph[0], p.d_line_d23, p.d_wgpart,
                            (V *)rho_out, (int64_t)p.ncand, mode == 0 ? p.d_gnum : nullptr);
       });
@@ -1751,15 +1759,20 @@ int launch_forward_front(Plan &p, const double *d_pos, void *W1, hipStream_t st)
                      : forward_front_t<double>(p, d_pos, W1, st);
 }
 
-int launch_forward_back(Plan &p, const void *W1, hipStream_t st, int mode, double *frame_out,
-                        void *rho_out) {
-  return p.prec == 1 ? forward_back_t<float>(p, W1, st, mode, frame_out, rho_out)
-                     : forward_back_t<double>(p, W1, st, mode, frame_out, rho_out);
+int launch_forward_y(Plan &p, const void *W1, void *W2, hipStream_t st) {
+  return p.prec == 1 ? forward_y_t<float>(p, W1, W2, st) : forward_y_t<double>(p, W1, W2, st);
+}
+
+int launch_forward_x(Plan &p, const void *W2, hipStream_t st, int mode, double *frame_out,
+                     void *rho_out) {
+  return p.prec == 1 ? forward_x_t<float>(p, W2, st, mode, frame_out, rho_out)
+                     : forward_x_t<double>(p, W2, st, mode, frame_out, rho_out);
 }
 
 int launch_forward(Plan &p, const double *d_pos, int mode, double *frame_out, void *rho_out) {
   if (int rc = launch_forward_front(p, d_pos, p.d_W1, p.stream)) return rc;
-  return launch_forward_back(p, p.d_W1, p.stream, mode, frame_out, rho_out);
+  if (int rc = launch_forward_y(p, p.d_W1, p.d_W2, p.stream)) return rc;
+  return launch_forward_x(p, p.d_W2, p.stream, mode, frame_out, rho_out);
 }
 
 template <typename T>
