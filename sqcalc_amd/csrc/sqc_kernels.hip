@@ -1407,7 +1407,6 @@ template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
                                                      const typename C2<T>::t *__restrict__ W1,
                                                      const typename C2<T>::t *__restrict__ twz,
-                                                     const int4 *__restrict__ i0a,
                                                      const int4 *__restrict__ i0s,
                                                      const T *__restrict__ phi,
                                                      const T *__restrict__ dphi,
@@ -1467,17 +1466,45 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   const int lpa = max(8, g.txr * g.ty), ga = 64 / lpa;
   const int grp = lane / lpa, l = lane - grp * lpa;
   T *scr = reinterpret_cast<T *>(rec + kStageCap) + (size_t)wave * ga * 6 * ns;
+  // The kernel values of a round's atoms, [atom][phi 3 ns, dphi 3 ns]: value
+  // k of the round is read by lane k % 64 into register k / 64.  When they
+  // fit kGV registers, the next round's are read while this one computes.
+  constexpr int kGV = 6;
+  const int nval = 6 * ns;
+  const float inv_nval = __builtin_amdgcn_rcpf((float)nval);
+  const bool ahead = ga * nval <= kGV * 64;
+  T nv[kGV];
+  auto fetch = [&](int s0, int na) {
+#pragma unroll
+    for (int u = 0; u < kGV; ++u) {
+      const int k = lane + 64 * u;
+      nv[u] = T(0);
+      if (k < na * nval) {
+        const int aa = small_div(k, inv_nval), kk = k - aa * nval;
+        const int j = rec[s0 + aa].x;
+        nv[u] = kk < 3 * ns ? phi[(size_t)j * 3 * ns + kk] : dphi[(size_t)j * 3 * ns + kk - 3 * ns];
+      }
+    }
+  };
   for (int base = 0; base < st.total; base += NT) {
     const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0s);
     const bool last = base + NT >= st.total;
     if (!last && nst <= kStageCap - NT) continue;
+    if (ahead && wave * ga < nst) fetch(wave * ga, min(ga, nst - wave * ga));
     for (int s0 = wave * ga; s0 < nst; s0 += NW * ga) {
       const int na = min(ga, nst - s0);
-      // phi then dphi of the round's atoms: [atom][6 ns]
-      for (int k = lane; k < na * 6 * ns; k += 64) {
-        const int aa = k / (6 * ns), kk = k - aa * 6 * ns;
-        const int j = rec[s0 + aa].x;
-        scr[k] = kk < 3 * ns ? phi[(size_t)j * 3 * ns + kk] : dphi[(size_t)j * 3 * ns + kk - 3 * ns];
+      if (ahead) {
+#pragma unroll
+        for (int u = 0; u < kGV; ++u)
+          if (lane + 64 * u < na * nval) scr[lane + 64 * u] = nv[u];
+        const int s1 = s0 + NW * ga;
+        if (s1 < nst) fetch(s1, min(ga, nst - s1));
+      } else {
+        for (int k = lane; k < na * nval; k += 64) {
+          const int aa = small_div(k, inv_nval), kk = k - aa * nval;
+          const int j = rec[s0 + aa].x;
+          scr[k] = kk < 3 * ns ? phi[(size_t)j * 3 * ns + kk] : dphi[(size_t)j * 3 * ns + kk - 3 * ns];
+        }
       }
       wave_sync();
       const bool act = grp < na;
@@ -1515,10 +1542,10 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
       }
       if (act && l == 0) {
         // slot of this (atom, tile) pair: tiles counted from the one that
-        // holds the first stencil point
+        // holds the first stencil point, which the record places at tile
+        // column xll - kxl (row yll - kyl), modulo the grid
         const int j = r.x;
-        const int4 i0 = i0a[j];
-        const int xs = wrapi(i0.x, n1), ys = wrapi(i0.y, n2);
+        const int xs = wrapi(x0 + xll - kxl, n1), ys = wrapi(y0 + yll - kyl, n2);
         const int dx = wrapi(tx - xs / g.txr, g.ntx);
         const int dy = wrapi(ty - ys / g.ty, g.nty);
         double *o = slots + ((size_t)j * S2 + dx * g.sy + dy) * 3;
@@ -1813,7 +1840,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
       hipLaunchKernelGGL((k_adj_z_gather<T, decltype(n)::value>), dim3(g.ntx * g.nty, Fa),
                          dim3(NT), p.lds1g, st, g, p.ax[2], (const V *)p.d_W1,
-                         (const V *)p.d_tw[2], p.d_i0, p.d_i0s, (const T *)p.d_phi,
+                         (const V *)p.d_tw[2], p.d_i0s, (const T *)p.d_phi,
                          (const T *)p.d_dphi,
                          p.d_sidx, p.d_start, p.d_slots);
     });
