@@ -728,6 +728,22 @@ __device__ __forceinline__ void spread_piece(T *tflat, T ph, T fxy, int cofs, in
   }
 }
 
+// Rotation by R lanes within each row of 16 lanes (DPP row_ror), and the
+// sum over a row that it gives every lane of the row.
+template <int R> __device__ __forceinline__ double row_ror(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x120 + R, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x120 + R, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double row16_sum(double v) {
+  v += row_ror<8>(v);
+  v += row_ror<4>(v);
+  v += row_ror<2>(v);
+  v += row_ror<1>(v);
+  return v;
+}
+
 // Lane shuffle through ds_bpermute with a byte address (4 * source lane)
 // computed once by the caller.
 __device__ __forceinline__ int bperm(int addr4, int v) { return __builtin_amdgcn_ds_bpermute(addr4, v); }
@@ -1524,21 +1540,34 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         const int xl = xll + ix, yl = yll + iy;
         const T *colv = tflat + 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1);
         T sv = T(0), sd = T(0);
-        int z = a;
-        for (int kz = 0; kz < ns; ++kz) {
-          const T v = colv[2 * z];
-          sv += v * ps[2 * ns + kz];
-          sd += v * ps[5 * ns + kz];
-          z = (z + 1 == n3) ? 0 : z + 1;
+        // z points a .. a + ns - 1: the run before the wrap, then the rest
+        const int n0 = min(ns, n3 - a);
+        const T *pz = ps + 2 * ns, *dz = ps + 5 * ns;
+        for (int kz = 0; kz < n0; ++kz) {
+          const T v = colv[2 * (a + kz)];
+          sv += v * pz[kz];
+          sd += v * dz[kz];
+        }
+        for (int kz = n0; kz < ns; ++kz) {
+          const T v = colv[2 * (kz - n0)];
+          sv += v * pz[kz];
+          sd += v * dz[kz];
         }
         gx = (double)(dfx * fy) * (double)sv;
         gy = (double)(fx * dfy) * (double)sv;
         gz = (double)(fx * fy) * (double)sd;
       }
-      for (int off = lpa >> 1; off > 0; off >>= 1) {
-        gx += __shfl_xor(gx, off);
-        gy += __shfl_xor(gy, off);
-        gz += __shfl_xor(gz, off);
+      if (lpa == 16) {
+        // an atom's lanes are one DPP row: rotations within the row
+        gx = row16_sum(gx);
+        gy = row16_sum(gy);
+        gz = row16_sum(gz);
+      } else {
+        for (int off = lpa >> 1; off > 0; off >>= 1) {
+          gx += __shfl_xor(gx, off);
+          gy += __shfl_xor(gy, off);
+          gz += __shfl_xor(gz, off);
+        }
       }
       if (act && l == 0) {
         // slot of this (atom, tile) pair: tiles counted from the one that
