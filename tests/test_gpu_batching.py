@@ -1,7 +1,11 @@
 """Batched frames and the tile autotuner.
 
-Graph replay: a plan with max_batch = 4 replays one captured hipGraph per
-four frames; a plan with max_batch = 1 launches every frame kernel by kernel.
+Graph replay: a plan with max_batch = B replays one captured hipGraph per
+B frames; a plan with max_batch = 1 launches every frame kernel by kernel.
+The graph runs the sort and spread of frame f + 1 beside the y and x passes
+of frame f, with the spread output alternating between two buffers, so B = 3
+(odd: the first frame of the next replay reuses the buffer of the last
+frame of this one) and B = 4 are both checked.
 The kernels and their order are the same; the fine grids agree bit for bit
 and only the order of the FP64 LDS additions inside a workgroup's shell
 accumulators may differ, so the accumulators must agree to 1e-12 relative (a
@@ -29,20 +33,20 @@ def _plan(cell, origin, types, precision, **kw):
                               eps=1e-10 if precision == "double" else None, **kw)
 
 
-@pytest.mark.parametrize("precision", ["double", "single"])
-def test_graph_replay_matches_eager(hip_device, precision):
+@pytest.mark.parametrize("precision,batch", [("double", 4), ("single", 4), ("single", 3)])
+def test_graph_replay_matches_eager(hip_device, precision, batch):
     import torch
     cell, origin, types, frames = random_case(natoms=1500, length=24.0, nframes=10, seed=31)
     pos = torch.tensor(frames, dtype=torch.float64, device=hip_device).contiguous()
-    print("case graph replay", precision, flush=True)
+    print("case graph replay", precision, batch, flush=True)
     eager = _plan(cell, origin, types, precision, max_batch=1)
     eager.accumulate_device(pos)
     ref_num, ref_part, _, ref_n = eager.fetch()
     assert ref_n == 10
-    graph = _plan(cell, origin, types, precision, max_batch=4)
+    graph = _plan(cell, origin, types, precision, max_batch=batch)
     for rep in range(3):
         graph.reset()
-        graph.accumulate_device(pos)          # 2 replays + 2 frames launched directly
+        graph.accumulate_device(pos)          # 10 // batch replays, the rest launched directly
         # eager work of the program between replays
         junk = (pos * 2.0).sum()
         torch.cuda.synchronize()
