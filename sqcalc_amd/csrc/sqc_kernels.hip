@@ -746,7 +746,6 @@ __device__ __forceinline__ double row16_sum(double v) {
 
 // Lane shuffle through ds_bpermute with a byte address (4 * source lane)
 // computed once by the caller.
-__device__ __forceinline__ int bperm(int addr4, int v) { return __builtin_amdgcn_ds_bpermute(addr4, v); }
 __device__ __forceinline__ float bperm(int addr4, float v) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr4, __float_as_int(v)));
 }
