@@ -1269,9 +1269,17 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
   V *rows = reinterpret_cast<V *>(smem);
   V *tw = rows + (size_t)lb * Fa * rs;
   double *inv1 = reinterpret_cast<double *>(tw + n1);
+  // per shell: G = gS / den (0 for an empty shell) and, with pair gradients,
+  // 1 / (N max(count, 1)), in the LDS the forward pass keeps its shell sums
+  // in (ncomp >= 2 rows of nq when there are pair gradients)
+  double *Gsh = inv1 + g.M[0] + 1, *inc = Gsh + nq;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
   for (int i = tid; i <= g.M[0]; i += NT) inv1[i] = 1.0 / ph1[i];
+  for (int i = tid; i < nq; i += NT) {
+    Gsh[i] = den[i] > 0.0 ? gS[i] / den[i] : 0.0;
+    if (gP) inc[i] = 1.0 / ((double)g.natoms * fmax(cnt[i], 1.0));
+  }
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int l0 = grp * lb;
     const int nl = min(lb, g.nlines - l0);
@@ -1321,19 +1329,19 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
           ti += w[a] * ri[a];
         }
       }
-      const double G = den[s] > 0.0 ? gS[s] / den[s] : 0.0;
+      const double G = Gsh[s];
 #pragma unroll
       for (int cf = 0; cf < FM && cf < Fa; ++cf) {
         const double wc = (Fa == 1) ? 1.0 : w[cf];
         double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
         if (gP) {
           // pair gradients exist only with one field per species
-          const double nc = (double)g.natoms * fmax(cnt[s], 1.0);
+          const double ic = inc[s];
 #pragma unroll
           for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
             for (int b = a; b < FM && b < F; ++b) {
-              const double H = gP[((size_t)a * S + b) * nq + s] / nc;
+              const double H = gP[((size_t)a * S + b) * nq + s] * ic;
               if (cf == a) { cr += H * rr[b]; ci -= H * ri[b]; }
               if (cf == b) { cr += H * rr[a]; ci -= H * ri[a]; }
             }
