@@ -76,14 +76,29 @@ def read_counter(path, counter):
 
 
 def read_trace(path):
-    """Dispatches of a --kernel-trace CSV grouped by (kernel, grid, LDS): the
-    tuner's trial launches then fall in groups of their own."""
-    groups = collections.OrderedDict()
+    """Dispatches of a --kernel-trace CSV grouped by (kernel, grid, LDS, alone):
+    the tuner's trial launches fall in groups of their own, and `alone` tells
+    the dispatches that ran by themselves from those whose run overlapped
+    another dispatch (the kernels of a captured batch that run side by side;
+    their start-to-end times include the sharing)."""
+    rows = []
     with open(path) as fh:
         for r in csv.DictReader(fh):
             grid = (int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]), int(r["Grid_Size_Z"]))
-            key = (r["Kernel_Name"], grid, int(r["LDS_Block_Size"]))
-            groups.setdefault(key, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], grid,
+                         int(r["LDS_Block_Size"])))
+    rows.sort()
+    overlapped = [False] * len(rows)
+    end_max, who = -1, -1
+    for i, (t0, t1, _, _, _) in enumerate(rows):
+        if t0 < end_max:
+            overlapped[i] = True
+            overlapped[who] = True
+        if t1 > end_max:
+            end_max, who = t1, i
+    groups = collections.OrderedDict()
+    for (t0, t1, name, grid, lds), ov in zip(rows, overlapped):
+        groups.setdefault((name, grid, lds, not ov), []).append(t1 - t0)
     return groups
 
 
@@ -99,15 +114,18 @@ def cmd_stats(args):
     if args.trace:
         groups = read_trace(args.trace)
         print("\nBy launch shape (the same kernels, dispatches grouped by grid and LDS "
-              "size; the tuner's trial tiles form groups of their own):\n")
-        print("| kernel | grid (threads) | LDS bytes | calls | average us |")
-        print("|---|---|---:|---:|---:|")
+              "size, and by whether the dispatch ran alone or overlapped another one, as "
+              "the kernels of consecutive frames do in a captured batch; the tuner's trial "
+              "tiles form groups of their own):\n")
+        print("| kernel | grid (threads) | LDS bytes | ran | calls | average us |")
+        print("|---|---|---:|---|---:|---:|")
         keep = [k for k in groups if base_name(k[0]) in STAGE_OF]
         keep.sort(key=lambda k: -sum(groups[k]))
         for k in keep[:args.top]:
             d = groups[k]
-            print("| `%s` | %s | %d | %d | %.2f |" % (short_name(k[0]), "x".join(map(str, k[1])),
-                                                      k[2], len(d), sum(d) / len(d) / 1e3))
+            print("| `%s` | %s | %d | %s | %d | %.2f |" % (short_name(k[0]), "x".join(map(str, k[1])),
+                                                           k[2], "alone" if k[3] else "overlapped",
+                                                           len(d), sum(d) / len(d) / 1e3))
 
 
 def cmd_traffic(args):
