@@ -77,10 +77,12 @@ typedef struct sqc_hip_config {
   const int32_t *species_of;  /* [natoms], 0-based species of each atom */
   const double *amp_const;    /* [nspecies] unit/neutron amplitudes */
   const double *xray_coef;    /* [nspecies][9] IT92 a1..a4, b1..b4, c */
-  int32_t tune;             /* 1 = time the spread tiles and y-pass chunks on the
-                               first frame and keep the fastest */
+  int32_t tune;             /* 1 = time the spread tiles, y-pass chunks and x-pass
+                               line groups on the first frame, keep the fastest */
   int32_t max_batch;        /* frames per captured hipGraph replay (0 = 8); 1 =
-                               launch every frame kernel by kernel */
+                               launch every frame kernel by kernel.  Above 1 the
+                               plan holds a second spread buffer: a replay spreads
+                               a frame while the previous one is transformed */
 } sqc_hip_config;
 
 typedef struct sqc_hip_plan sqc_hip_plan;
