@@ -634,7 +634,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
         }
     if (pick < 0) return fail(SQC_HIP_ERR_MAXNALLOC, "no spread tile fits the LDS");
     int tx2 = 16;
-    while (tx2 > 1 && ypass_lds(p, tx2) > lds_pref_y(p)) tx2 /= 2;
+    while (tx2 > 2 && ypass_lds(p, tx2) > lds_pref_y(p)) tx2 /= 2;  // the y pass is compiled for 2..16
     set_tiles(p, kTiles[pick][0], kTiles[pick][1], tx2);
   }
   int lb0 = sqc::kMaxLb;

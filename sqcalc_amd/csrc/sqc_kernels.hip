@@ -957,8 +957,10 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
 // y-DFT.  One workgroup per (x-chunk, h3, field); lines are transposed into
 // contiguous LDS rows (stride n2 + 1) and transformed one per wave.
 // ---------------------------------------------------------------------------
-template <typename T, int N>
-__global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
+// TX2: the column chunk at compile time (the tuner picks it among 2, 4, 8
+// and 16), so that a thread's column and row steps are constants.
+template <typename T, int N, int TX2>
+__global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay,
                                               const typename C2<T>::t *__restrict__ W1,
                                               const typename C2<T>::t *__restrict__ twy,
                                               const int32_t *__restrict__ lo2,
@@ -969,6 +971,8 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
   extern __shared__ __align__(16) unsigned char smem[];
   const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
   V *lines = reinterpret_cast<V *>(smem);
+  constexpr int tx2 = TX2, lt = TX2 == 16 ? 4 : TX2 == 8 ? 3 : TX2 == 4 ? 2 : 1, tm = TX2 - 1;
+  static_assert(TX2 == 2 || TX2 == 4 || TX2 == 8 || TX2 == 16, "column chunk");
   const int nbx = (g.n[0] + tx2 - 1) / tx2;
   const int blk = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
   const int x0 = (blk % nbx) * tx2, k3 = (blk / nbx) % (g.M[2] + 1), f = blk / (nbx * (g.M[2] + 1));
@@ -976,7 +980,6 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay, int tx2,
   const int lo = lo2[k3], hi = hi2[k3];
   if (hi < lo) return;
   const V *src = W1 + ((size_t)f * (M3 + 1) + k3) * n2 * n1;
-  const int lt = g.ltx2, tm = tx2 - 1;
   batched_copy<8, V>(
       n2 * tx2,
       [&](int idx) {
@@ -1720,6 +1723,14 @@ template <> struct FixedSizes<double> {
   }
 };
 
+// The y pass's column chunk as a compile-time constant (2, 4, 8 or 16).
+template <typename Fn> static void pick_chunk(int tx2, Fn &&fn) {
+  if (tx2 == 16) return fn(std::integral_constant<int, 16>{});
+  if (tx2 == 8) return fn(std::integral_constant<int, 8>{});
+  if (tx2 == 4) return fn(std::integral_constant<int, 4>{});
+  fn(std::integral_constant<int, 2>{});
+}
+
 // Field counts the x-pass epilogue is compiled for; others run its loops
 // with the count read at run time.
 template <typename Fn> static void pick_fields(int F, Fn &&fn) {
@@ -1775,10 +1786,12 @@ static int forward_y_t(Plan &p, const void *W1, void *W2, hipStream_t st) {
   {
     StageTimer t(p, 2);
     FixedSizes<T>::pick(g.n[1], [&](auto n) {
-      hipLaunchKernelGGL((k_fft_y<T, decltype(n)::value>),
-                         dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * g.F), dim3(NT), p.lds2, st,
-                         g, p.ax[1], p.tx2, (const V *)W1, (const V *)p.d_tw[1], p.d_lo2,
-                         p.d_hi2, p.d_loff, (V *)W2);
+      pick_chunk(p.tx2, [&](auto c) {
+        hipLaunchKernelGGL((k_fft_y<T, decltype(n)::value, decltype(c)::value>),
+                           dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * g.F), dim3(NT),
+                           p.lds2, st, g, p.ax[1], (const V *)W1, (const V *)p.d_tw[1], p.d_lo2,
+                           p.d_hi2, p.d_loff, (V *)W2);
+      });
     });
   }
   SQC_CHECK(hipGetLastError());
@@ -1914,7 +1927,9 @@ static int lds_limits_t(Plan &p) {
     set((const void *)k_adj_z_gather<T, decltype(n)::value>, p.lds1g);
   });
   FixedSizes<T>::pick(p.geo.n[1], [&](auto n) {
-    set((const void *)k_fft_y<T, decltype(n)::value>, p.lds2);
+    pick_chunk(p.tx2, [&](auto c) {
+      set((const void *)k_fft_y<T, decltype(n)::value, decltype(c)::value>, p.lds2);
+    });
     set((const void *)k_adj_y<T, decltype(n)::value>, p.lds2);
   });
   FixedSizes<T>::pick(p.geo.n[0], [&](auto n) {
