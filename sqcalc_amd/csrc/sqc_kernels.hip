@@ -408,6 +408,15 @@ __device__ __forceinline__ int xcd_block(int i, int nb, int nxcd) {
 // offset keeps c / d at least 1 / (2 d) away from an integer.
 __device__ __forceinline__ int small_div(int c, float inv_d) { return (int)(((float)c + 0.5f) * inv_d); }
 
+// c / D for a non-negative c: by the constant D when it is known at compile
+// time (D > 0), else through the float reciprocal inv_d of the run-time d.
+template <int D> __device__ __forceinline__ int div_by(int c, float inv_d) {
+  if constexpr (D > 0)
+    return (int)((unsigned)c / (unsigned)D);
+  else
+    return small_div(c, inv_d);
+}
+
 // ---------------------------------------------------------------------------
 // Atom preparation and counting sort
 // ---------------------------------------------------------------------------
@@ -1111,12 +1120,12 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
     batched_copy<8, V>(
         nl * F * n1,
         [&](int idx) {
-          const int r = small_div(idx, inv_n1), x = idx - r * n1;
-          const int li = small_div(r, inv_f), f = r - li * F;
+          const int r = div_by<N>(idx, inv_n1), x = idx - r * n1;
+          const int li = div_by<FC>(r, inv_f), f = r - li * F;
           return W2[(unsigned)((f * g.nlines + l0 + li) * n1 + x)];
         },
         [&](int idx, V v) {
-          const int r = small_div(idx, inv_n1);
+          const int r = div_by<N>(idx, inv_n1);
           rows[r * rs + (idx - r * n1)] = v;
         });
     gl.put(lg, nl);
@@ -1374,8 +1383,8 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
     __syncthreads();
     const float inv_n1 = 1.0f / (float)n1, inv_fa = 1.0f / (float)Fa;
     for (int idx = tid; idx < nl * Fa * n1; idx += NT) {
-      const int r = small_div(idx, inv_n1), x = idx - r * n1;
-      const int li = small_div(r, inv_fa), cf = r - li * Fa;
+      const int r = div_by<N>(idx, inv_n1), x = idx - r * n1;
+      const int li = div_by<FC>(r, inv_fa), cf = r - li * Fa;
       W2[(unsigned)((cf * g.nlines + l0 + li) * n1 + x)] = rows[r * rs + x];
     }
   }
