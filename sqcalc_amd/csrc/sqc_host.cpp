@@ -337,7 +337,6 @@ void set_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
   g.sx = std::min(g.ntx, (ns - 1 + g.txr - 1) / g.txr + 2);
   g.sy = std::min(g.nty, (ns - 1 + g.ty - 1) / g.ty + 2);
   p.tx2 = tx2;
-  g.ltx2 = ilog2i(tx2);
   g.ls2 = ystride(p, tx2);
   p.lds2 = ypass_lds(p, tx2);
 }

@@ -57,7 +57,7 @@ struct Geo {
   double half_ns;
   // spread tiles (real x-columns x rows) and the bins that sort atoms by tile
   int txr, ty, ntx, nty, nbins;
-  int ltxr, lty, ltx2;  // log2 of txr, ty and of the y-pass column chunk
+  int ltxr, lty;     // log2 of txr and ty
   int cs;           // LDS stride of a z column in the spread tile (padded)
   int ls2;          // LDS stride of a y line (padded)
   int rs1;          // LDS stride of an x row (padded)

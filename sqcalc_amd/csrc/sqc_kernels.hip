@@ -1390,8 +1390,8 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
   }
 }
 
-template <typename T, int N>
-__global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
+template <typename T, int N, int TX2>
+__global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay,
                                               const typename C2<T>::t *__restrict__ W2,
                                               const typename C2<T>::t *__restrict__ twy,
                                               const int32_t *__restrict__ lo2,
@@ -1402,6 +1402,8 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
   extern __shared__ __align__(16) unsigned char smem[];
   const int n1 = g.n[0], n2 = g.n[1], M3 = g.M[2], ls = g.ls2;
   V *lines = reinterpret_cast<V *>(smem);
+  constexpr int tx2 = TX2, lt = TX2 == 16 ? 4 : TX2 == 8 ? 3 : TX2 == 4 ? 2 : 1, tm = TX2 - 1;
+  static_assert(TX2 == 2 || TX2 == 4 || TX2 == 8 || TX2 == 16, "column chunk");
   const int nbx = (g.n[0] + tx2 - 1) / tx2;
   const int blk = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
   const int x0 = (blk % nbx) * tx2, k3 = (blk / nbx) % (g.M[2] + 1), cf = blk / (nbx * (g.M[2] + 1));
@@ -1410,7 +1412,6 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay, int tx2,
   for (int i = tid; i < tx2 * ls; i += NT) lines[i] = V{T(0), T(0)};
   __syncthreads();
   const V *src = W2 + ((size_t)cf * g.nlines + (hi >= lo ? loff[k3] : 0)) * n1;
-  const int lt = g.ltx2, tm = tx2 - 1;
   batched_copy<8, V>(
       (hi - lo + 1) * tx2,
       [&](int idx) {
@@ -1887,10 +1888,12 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   {
     StageTimer t(p, 6);
     FixedSizes<T>::pick(g.n[1], [&](auto n) {
-      hipLaunchKernelGGL((k_adj_y<T, decltype(n)::value>),
-                         dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * Fa), dim3(NT), p.lds2, st,
-                         g, p.ax[1], p.tx2, (const V *)p.d_W2, (const V *)p.d_tw[1], p.d_lo2,
-                         p.d_hi2, p.d_loff, (V *)p.d_W1);
+      pick_chunk(p.tx2, [&](auto c) {
+        hipLaunchKernelGGL((k_adj_y<T, decltype(n)::value, decltype(c)::value>),
+                           dim3((g.n[0] + p.tx2 - 1) / p.tx2 * (g.M[2] + 1) * Fa), dim3(NT),
+                           p.lds2, st, g, p.ax[1], (const V *)p.d_W2, (const V *)p.d_tw[1],
+                           p.d_lo2, p.d_hi2, p.d_loff, (V *)p.d_W1);
+      });
     });
   }
   {
@@ -1939,7 +1942,9 @@ static int lds_limits_t(Plan &p) {
     pick_chunk(p.tx2, [&](auto c) {
       set((const void *)k_fft_y<T, decltype(n)::value, decltype(c)::value>, p.lds2);
     });
-    set((const void *)k_adj_y<T, decltype(n)::value>, p.lds2);
+    pick_chunk(p.tx2, [&](auto c) {
+      set((const void *)k_adj_y<T, decltype(n)::value, decltype(c)::value>, p.lds2);
+    });
   });
   FixedSizes<T>::pick(p.geo.n[0], [&](auto n) {
     pick_fields(p.geo.F, [&](auto fc) {
