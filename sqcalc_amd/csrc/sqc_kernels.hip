@@ -828,7 +828,7 @@ __device__ __forceinline__ void spread_whole(T *tflat, T ph, T w, int ox, int kx
 // and walks the staged atoms in order, so every LDS cell is updated by one
 // wave, in atom order: no atomics, and the grid is bit-reproducible.
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+template <typename T, int N, int TT>
 __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 *__restrict__ i0s,
                                                  const T *__restrict__ phi,
                                                  const int32_t *__restrict__ sidx,
@@ -840,7 +840,12 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Stage st;
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], cs = g.cs;
-  const int ccx = g.txr / 2, cc = ccx * g.ty;
+  // the tile shape: compiled in for the common 8 x 2 tile (TT = 1), read
+  // at run time otherwise
+  constexpr bool kt = TT == 1;
+  const int txr = kt ? 8 : g.txr, tty = kt ? 2 : g.ty;
+  const int ltxr = kt ? 3 : g.ltxr, ltyy = kt ? 1 : g.lty;
+  const int ccx = txr / 2, cc = ccx * tty;
   V *tile = reinterpret_cast<V *>(smem);
   int4 *rec = reinterpret_cast<int4 *>(tile + (size_t)cc * cs);
   T *tflat = reinterpret_cast<T *>(tile);
@@ -848,8 +853,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   // XCD-contiguous renumbering the y pass uses
   const int f = blockIdx.y, tb = blockIdx.x;
   const int tx = tb % g.ntx, ty = tb / g.ntx;
-  const int x0 = tx * g.txr, y0 = ty * g.ty;
-  const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
+  const int x0 = tx * txr, y0 = ty * tty;
+  const int txw = min(txr, n1 - x0), tyw = min(tty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
   stage_window(st, g, tx, ty, f, start);
@@ -944,8 +949,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   // split the packed pairs and write h3 = 0..M3.  txr * ty is a power of two
   // that divides NT, so every thread keeps one (xl, yl) column and steps h3.
   const int M3 = g.M[2];
-  const int lper = g.ltxr + g.lty;
-  const int xl = tid & (g.txr - 1), yl = (tid >> g.ltxr) & (g.ty - 1);
+  const int lper = ltxr + ltyy;
+  const int xl = tid & (txr - 1), yl = (tid >> ltxr) & (tty - 1);
   if (xl >= txw || yl >= tyw) return;
   const int kstep = NT >> lper;
   const V *colp = tile + (yl * ccx + (xl >> 1)) * cs;
@@ -1780,9 +1785,16 @@ static int forward_front_t(Plan &p, const double *d_pos, void *W1, hipStream_t s
   {
     StageTimer t(p, 1);
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
-      hipLaunchKernelGGL((k_spread_z<T, decltype(n)::value>), dim3(g.ntx * g.nty, g.F), dim3(NT),
-                         p.lds1, st, g, p.ax[2], p.d_i0s, (const T *)p.d_phi, p.d_sidx, p.d_start,
-                         p.d_species, (const V *)p.d_tw[2], (V *)W1);
+      auto go = [&](auto tt) {
+        hipLaunchKernelGGL((k_spread_z<T, decltype(n)::value, decltype(tt)::value>),
+                           dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2], p.d_i0s,
+                           (const T *)p.d_phi, p.d_sidx, p.d_start, p.d_species,
+                           (const V *)p.d_tw[2], (V *)W1);
+      };
+      if (g.txr == 8 && g.ty == 2)
+        go(std::integral_constant<int, 1>{});
+      else
+        go(std::integral_constant<int, 0>{});
     });
   }
   SQC_CHECK(hipGetLastError());
@@ -1935,7 +1947,8 @@ static int lds_limits_t(Plan &p) {
     }
   };
   FixedSizes<T>::pick(p.geo.n[2], [&](auto n) {
-    set((const void *)k_spread_z<T, decltype(n)::value>, p.lds1);
+    set((const void *)k_spread_z<T, decltype(n)::value, 0>, p.lds1);
+    set((const void *)k_spread_z<T, decltype(n)::value, 1>, p.lds1);
     set((const void *)k_adj_z_gather<T, decltype(n)::value>, p.lds1g);
   });
   FixedSizes<T>::pick(p.geo.n[1], [&](auto n) {
