@@ -1444,7 +1444,7 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay,
   }
 }
 
-template <typename T, int N>
+template <typename T, int N, int TT>
 __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
                                                      const typename C2<T>::t *__restrict__ W1,
                                                      const typename C2<T>::t *__restrict__ twz,
@@ -1458,15 +1458,19 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Stage st;
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2], cs = g.cs;
-  const int ccx = g.txr / 2, cc = ccx * g.ty;
+  // the tile shape: compiled in for the common 8 x 2 tile (TT = 1), read
+  // at run time otherwise
+  constexpr bool kt = TT == 1;
+  const int txr = kt ? 8 : g.txr, tty = kt ? 2 : g.ty;
+  const int ccx = txr / 2, cc = ccx * tty;
   V *tile = reinterpret_cast<V *>(smem);
   int4 *rec = reinterpret_cast<int4 *>(tile + (size_t)cc * cs);
   const T *tflat = reinterpret_cast<const T *>(tile);
   // grid (tiles, fields), as the spreader's
   const int cf = blockIdx.y, tb = blockIdx.x;
   const int tx = tb % g.ntx, ty = tb / g.ntx;
-  const int x0 = tx * g.txr, y0 = ty * g.ty;
-  const int txw = min(g.txr, n1 - x0), tyw = min(g.ty, n2 - y0);
+  const int x0 = tx * txr, y0 = ty * tty;
+  const int txw = min(txr, n1 - x0), tyw = min(tty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
   __syncthreads();
@@ -1504,7 +1508,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   // atoms go to a per-wave scratch; each lane sums its column over z in the
   // path precision, and the three derivative sums are reduced in FP64 over
   // the atom's lanes.
-  const int lpa = max(8, g.txr * g.ty), ga = 64 / lpa;
+  const int lpa = max(8, txr * tty), ga = 64 / lpa;
   const int grp = lane / lpa, l = lane - grp * lpa;
   T *scr = reinterpret_cast<T *>(rec + kStageCap) + (size_t)wave * ga * 6 * ns;
   // The kernel values of a round's atoms, [atom][phi 3 ns, dphi 3 ns]: value
@@ -1600,8 +1604,8 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         // column xll - kxl (row yll - kyl), modulo the grid
         const int j = r.x;
         const int xs = wrapi(x0 + xll - kxl, n1), ys = wrapi(y0 + yll - kyl, n2);
-        const int dx = wrapi(tx - xs / g.txr, g.ntx);
-        const int dy = wrapi(ty - ys / g.ty, g.nty);
+        const int dx = wrapi(tx - xs / txr, g.ntx);
+        const int dy = wrapi(ty - ys / tty, g.nty);
         double *o = slots + ((size_t)j * S2 + dx * g.sy + dy) * 3;
         o[0] = gx;
         o[1] = gy;
@@ -1911,11 +1915,17 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   {
     StageTimer t(p, 7);
     FixedSizes<T>::pick(g.n[2], [&](auto n) {
-      hipLaunchKernelGGL((k_adj_z_gather<T, decltype(n)::value>), dim3(g.ntx * g.nty, Fa),
-                         dim3(NT), p.lds1g, st, g, p.ax[2], (const V *)p.d_W1,
-                         (const V *)p.d_tw[2], p.d_i0s, (const T *)p.d_phi,
-                         (const T *)p.d_dphi,
-                         p.d_sidx, p.d_start, p.d_slots);
+      auto go = [&](auto tt) {
+        hipLaunchKernelGGL((k_adj_z_gather<T, decltype(n)::value, decltype(tt)::value>),
+                           dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1g, st, g, p.ax[2],
+                           (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_i0s,
+                           (const T *)p.d_phi, (const T *)p.d_dphi, p.d_sidx, p.d_start,
+                           p.d_slots);
+      };
+      if (g.txr == 8 && g.ty == 2)
+        go(std::integral_constant<int, 1>{});
+      else
+        go(std::integral_constant<int, 0>{});
     });
   }
   {
@@ -1949,7 +1959,8 @@ static int lds_limits_t(Plan &p) {
   FixedSizes<T>::pick(p.geo.n[2], [&](auto n) {
     set((const void *)k_spread_z<T, decltype(n)::value, 0>, p.lds1);
     set((const void *)k_spread_z<T, decltype(n)::value, 1>, p.lds1);
-    set((const void *)k_adj_z_gather<T, decltype(n)::value>, p.lds1g);
+    set((const void *)k_adj_z_gather<T, decltype(n)::value, 0>, p.lds1g);
+    set((const void *)k_adj_z_gather<T, decltype(n)::value, 1>, p.lds1g);
   });
   FixedSizes<T>::pick(p.geo.n[1], [&](auto n) {
     pick_chunk(p.tx2, [&](auto c) {
