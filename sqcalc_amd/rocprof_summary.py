@@ -58,9 +58,9 @@ def read_stats(path):
 
 
 def read_counter(path, counter):
-    """Mean counter value per dispatch, per kernel name, over the kernel's most
-    frequent launch shape (grid and LDS size): the tuner's trial launches of
-    other tiles are left out."""
+    """Mean counter value per dispatch and the number of dispatches, per
+    kernel name, over the kernel's most frequent launch shape (grid and LDS
+    size): the tuner's trial launches of other tiles are left out."""
     acc = collections.defaultdict(lambda: collections.defaultdict(list))
     with open(path) as fh:
         for r in csv.DictReader(fh):
@@ -71,7 +71,7 @@ def read_counter(path, counter):
     out = {}
     for k, shapes in acc.items():
         v = max(shapes.values(), key=len)
-        out[k] = sum(v) / len(v)
+        out[k] = (sum(v) / len(v), len(v))
     return out
 
 
@@ -131,8 +131,8 @@ def cmd_stats(args):
 def cmd_traffic(args):
     fetch = read_counter(args.fetch, "FETCH_SIZE")
     write = read_counter(args.write, "WRITE_SIZE")
-    copies_f = [v for k, v in fetch.items() if is_copy(k)]
-    copies_w = [v for k, v in write.items() if is_copy(k)]
+    copies_f = [v for k, (v, _) in fetch.items() if is_copy(k)]
+    copies_w = [v for k, (v, _) in write.items() if is_copy(k)]
     if not copies_f or not copies_w:
         sys.exit("no calibration copy kernel in the counter files")
     kib = args.copy_bytes / 1024.0
@@ -145,19 +145,30 @@ def cmd_traffic(args):
     print("Calibration: elementwise copy of %d bytes in and out read FETCH_SIZE %.0f KiB, "
           "WRITE_SIZE %.0f KiB; scale factors fetch %.4f, write %.4f.\n"
           % (args.copy_bytes, copies_f[0], copies_w[0], sf, sw))
-    print("| kernel | stage | fetch MB | write MB | total MB (scaled) |")
-    print("|---|---|---:|---:|---:|")
-    per_stage = collections.defaultdict(float)
-    for k in sorted(fetch, key=lambda k: -fetch[k]):
+    print("| kernel | stage | dispatches | fetch MB | write MB | total MB (scaled) |")
+    print("|---|---|---:|---:|---:|---:|")
+    # a stage's bytes: the sum over its kernels, each taken in the compiled
+    # instance that ran most often (the tuned plan's; the tuner's trials of
+    # other tiles run other instances)
+    chosen = {}
+    for k, (_, n) in fetch.items():
         b = base_name(k)
         if b not in STAGE_OF or k not in write:
             continue
-        fb = fetch[k] * 1024.0 * sf
-        wb = write[k] * 1024.0 * sw
+        if b not in chosen or n > fetch[chosen[b]][1]:
+            chosen[b] = k
+    per_stage = collections.defaultdict(float)
+    for k in sorted(fetch, key=lambda k: -fetch[k][0]):
+        b = base_name(k)
+        if b not in STAGE_OF or k not in write:
+            continue
+        fb = fetch[k][0] * 1024.0 * sf
+        wb = write[k][0] * 1024.0 * sw
         st = STAGE_OF[b]
-        per_stage[st] += fb + wb
-        print("| `%s` | %s | %.1f | %.1f | %.1f |" % (short_name(k), st, fb / 1e6, wb / 1e6,
-                                                     (fb + wb) / 1e6))
+        if chosen[b] == k:
+            per_stage[st] += fb + wb
+        print("| `%s` | %s | %d | %.1f | %.1f | %.1f |" % (short_name(k), st, fetch[k][1], fb / 1e6,
+                                                          wb / 1e6, (fb + wb) / 1e6))
     for st, v in per_stage.items():
         out[st] = int(round(v))
     if args.json:
