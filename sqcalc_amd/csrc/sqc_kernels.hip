@@ -1088,9 +1088,10 @@ __device__ __forceinline__ void load_group(LineGroup &lg, int l0, int nl,
 }
 
 // FC: the number of fields when the epilogue is compiled for it (0: g.F at
-// run time).  With more than one field, fields and species coincide.
-template <typename T, int N, int FC>
-__global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, int ngroups,
+// run time).  With more than one field, fields and species coincide.  LB:
+// the lines per group when compiled in (2, the common tuned value; 0: lb_).
+template <typename T, int N, int FC, int LB>
+__global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, int ngroups,
                                                   const typename C2<T>::t *__restrict__ W2,
                                                   const typename C2<T>::t *__restrict__ twx,
                                                   const int4 *__restrict__ line_info,
@@ -1102,6 +1103,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb, i
                                                   typename C2<T>::t *__restrict__ rho_out,
                                                   int64_t ncand, double *__restrict__ gnum) {
   using V = typename C2<T>::t;
+  const int lb = LB > 0 ? LB : lb_;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LineGroup lg;
   const int n1 = g.n[0], F = FC > 0 ? FC : g.F, nq = g.nq, ncomp = g.ncomp, M1 = g.M[0];
@@ -1834,11 +1836,18 @@ static int forward_x_t(Plan &p, const void *W2, hipStream_t st, int mode, double
     StageTimer t(p, 3);
     FixedSizes<T>::pick(g.n[0], [&](auto n) {
       pick_fields(g.F, [&](auto fc) {
-        hipLaunchKernelGGL((k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>),
-                           dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups,
-                           (const V *)W2, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
-                           p.d_line_off, p.d_stab, p.d_ph[0], p.d_line_d23, p.d_wgpart,
-                           (V *)rho_out, (int64_t)p.ncand, mode == 0 ? p.d_gnum : nullptr);
+        auto go = [&](auto lbc) {
+          hipLaunchKernelGGL(
+              (k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, decltype(lbc)::value>),
+              dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups, (const V *)W2,
+              (const V *)p.d_tw[0], (const int4 *)p.d_line_info, p.d_line_off, p.d_stab,
+              p.d_ph[0], p.d_line_d23, p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
+              mode == 0 ? p.d_gnum : nullptr);
+        };
+        if (p.lb == 2)
+          go(std::integral_constant<int, 2>{});
+        else
+          go(std::integral_constant<int, 0>{});
       });
     });
   }
@@ -1972,7 +1981,8 @@ static int lds_limits_t(Plan &p) {
   });
   FixedSizes<T>::pick(p.geo.n[0], [&](auto n) {
     pick_fields(p.geo.F, [&](auto fc) {
-      set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>, p.lds3);
+      set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, 0>, p.lds3);
+      set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, 2>, p.lds3);
     });
     pick_fields(p.geo.F, [&](auto fc) {
       set((const void *)k_adj_x<T, decltype(n)::value, decltype(fc)::value>, p.lds3);
@@ -1990,10 +2000,16 @@ static void persistent_grids_t(Plan &p) {
   int nx = 0;
   FixedSizes<T>::pick(g.n[0], [&](auto n) {
     pick_fields(g.F, [&](auto fc) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nx, k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value>, NT, p.lds3) !=
-          hipSuccess)
-        nx = 0;
+      auto query = [&](auto lbc) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nx, k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, decltype(lbc)::value>,
+                NT, p.lds3) != hipSuccess)
+          nx = 0;
+      };
+      if (p.lb == 2)
+        query(std::integral_constant<int, 2>{});
+      else
+        query(std::integral_constant<int, 0>{});
     });
   });
   (void)hipGetLastError();
