@@ -856,7 +856,13 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int x0 = tx * txr, y0 = ty * tty;
   const int txw = min(txr, n1 - x0), tyw = min(tty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
+  {
+    // clear the tile 16 bytes a store (cc * cs * sizeof(V) is a multiple of
+    // 16: cs is even)
+    float4 *t4 = reinterpret_cast<float4 *>(tile);
+    const int n4 = cc * cs * (int)sizeof(V) / 16;
+    for (int i = tid; i < n4; i += NT) t4[i] = float4{0.f, 0.f, 0.f, 0.f};
+  }
   stage_window(st, g, tx, ty, f, start);
 
   const int zs0 = (n3 * wave) / NW, zs1 = (n3 * (wave + 1)) / NW;
@@ -1474,7 +1480,13 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   const int x0 = tx * txr, y0 = ty * tty;
   const int txw = min(txr, n1 - x0), tyw = min(tty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int i = tid; i < cc * cs; i += NT) tile[i] = V{T(0), T(0)};
+  {
+    // clear the tile 16 bytes a store (cc * cs * sizeof(V) is a multiple of
+    // 16: cs is even)
+    float4 *t4 = reinterpret_cast<float4 *>(tile);
+    const int n4 = cc * cs * (int)sizeof(V) / 16;
+    for (int i = tid; i < n4; i += NT) t4[i] = float4{0.f, 0.f, 0.f, 0.f};
+  }
   __syncthreads();
   const int pw = txw / 2;
   const V *src = W1 + (size_t)cf * (M3 + 1) * n2 * n1;
