@@ -906,13 +906,13 @@ int tune(sqc::Plan &p, const double *pos_dev) {
 int capture_batch(sqc::Plan &p) {
   const int64_t stride = 3 * p.geo.natoms;
   const int64_t B = p.batch_cap;
-  const size_t nev = 2 * (size_t)B + 1;
+  const size_t nev = 3 * (size_t)B + 1;
   while (p.gev.size() < nev) {
     hipEvent_t e = nullptr;
     HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     p.gev.push_back(e);
   }
-  hipEvent_t *evS = p.gev.data() + 1, *evB = evS + B;
+  hipEvent_t *evS = p.gev.data() + 1, *evY = evS + B, *evB = evY + B;
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   int rc = 0;
@@ -926,11 +926,12 @@ int capture_batch(sqc::Plan &p) {
   step([&] { return hipStreamWaitEvent(p.stream2, p.gev[0], 0); });
   for (int64_t f = 0; f < B && !rc; ++f) {
     void *W1 = (f & 1) ? p.d_W1b : p.d_W1;
-    if (f >= 2) step([&] { return hipStreamWaitEvent(p.stream, evB[f - 2], 0); });  // W1 reuse
+    if (f >= 2) step([&] { return hipStreamWaitEvent(p.stream, evY[f - 2], 0); });  // W1 reuse
     if (!rc) rc = sqc::launch_forward_front(p, p.d_pos + f * stride, W1, p.stream);
     step([&] { return hipEventRecord(evS[f], p.stream); });
     step([&] { return hipStreamWaitEvent(p.stream2, evS[f], 0); });
     if (!rc) rc = sqc::launch_forward_y(p, W1, p.d_W2, p.stream2);
+    step([&] { return hipEventRecord(evY[f], p.stream2); });
     if (!rc) rc = sqc::launch_forward_x(p, p.d_W2, p.stream2, 0, nullptr, nullptr);
     step([&] { return hipEventRecord(evB[f], p.stream2); });
   }
