@@ -184,6 +184,77 @@ const char *sqc_hip_last_error(void);
  * reference's opts ABI check (src/sqc_cufinufft.f90:277-290). */
 size_t sqc_hip_config_size(void);
 
+/* ---- LAMMPS dump reader (host side, no device needed) ----
+ *
+ * Replaces the reference's trajectory reader lammps_dump_reader_t
+ * (src/sqc_dump.f90:72-283: open, next_frame, close) and the driver's
+ * per-frame checks (app/main.f90:240-247), and feeds sqc_hip_accumulate
+ * directly: sqc_dump_read_frames writes a batch of frames in its pos layout.
+ * Frames follow the reference reader: header records, BOX BOUNDS to the
+ * restricted triclinic cell (src/sqc_cell.f90:39-63), positions unwrapped
+ * across frames unless the dump has xu yu zu (src/sqc_dump.f90:300-364).
+ * The scaled columns xs ys zs / xsu ysu zsu are accepted as well.  The file
+ * is memory-mapped and each atom block is parsed by nthreads threads. */
+
+/* Status codes: 0 and 1..11 are the reference reader's ierr values. */
+enum {
+  SQC_DUMP_OK = 0,
+  SQC_DUMP_END = 1,                /* clean end of the trajectory */
+  SQC_DUMP_ERR_OPEN = 2,
+  SQC_DUMP_ERR_CLOSED = 3,
+  SQC_DUMP_ERR_HEADER = 4,         /* no TIMESTEP where a frame starts, bad step, early end */
+  SQC_DUMP_ERR_NATOMS = 5,         /* NUMBER OF ATOMS not a positive integer */
+  SQC_DUMP_ERR_BOX = 6,            /* BOX BOUNDS row with fewer than two numbers */
+  SQC_DUMP_ERR_NO_ATOMS = 7,       /* frame without BOX BOUNDS */
+  SQC_DUMP_ERR_ATOM_LINE = 8,      /* atom line missing or not all-numeric */
+  SQC_DUMP_ERR_COLUMNS = 9,        /* ATOMS line without type or coordinates */
+  SQC_DUMP_ERR_ORDER = 10,         /* ids change order between frames */
+  SQC_DUMP_ERR_COUNT = 11,         /* atom count changes (unwrapping state) */
+  SQC_DUMP_ERR_NATOMS_CHANGED = 12, /* app/main.f90:240-243 (stop 8) */
+  SQC_DUMP_ERR_BOX_CHANGED = 13,   /* app/main.f90:244-247 (stop 9) */
+  SQC_DUMP_ERR_ARG = 14
+};
+
+typedef struct sqc_dump_reader sqc_dump_reader;
+
+/* One frame's header: frame_t of src/sqc_dump.f90:22-45. */
+typedef struct sqc_dump_frame {
+  int64_t natoms;
+  int64_t timestep;
+  double cell[9];            /* lattice vectors in the columns (cell_t%a) */
+  double origin[3];
+  int32_t pbc[3];            /* 1 where the BOX BOUNDS flag starts with 'p' */
+  int32_t triclinic;         /* the BOX BOUNDS line named xy */
+  int32_t scaled;            /* coordinates came from xs ys zs (or xsu ysu zsu) */
+  int32_t has_id;
+  int32_t unwrapped;         /* always 1: positions are continuous */
+  int32_t unwrapped_from_dump;
+  int64_t n_ambiguous;       /* atom steps above 0.4 of a cell vector so far */
+  int64_t lineno;            /* lines read so far */
+  int64_t frames;            /* frames read so far */
+} sqc_dump_frame;
+
+/* nthreads <= 0: OMP_NUM_THREADS, else the hardware thread count. */
+int sqc_dump_open(sqc_dump_reader **reader, const char *path, int nthreads);
+/* Next frame into the reader's arrays; SQC_DUMP_END at the end.  After an
+ * error the reader returns that error from then on. */
+int sqc_dump_next(sqc_dump_reader *reader, sqc_dump_frame *frame);
+/* The last sqc_dump_next frame: pos[natoms][3], id[natoms], type_id[natoms]
+ * (any may be NULL). */
+int sqc_dump_copy(const sqc_dump_reader *reader, double *pos, int64_t *id, int32_t *type_id);
+/* Up to maxframes frames of natoms atoms into pos[maxframes][natoms][3],
+ * the layout of sqc_hip_accumulate.  *nread < maxframes means the end of the
+ * file.  Each frame must keep natoms atoms and, when cell is non-null, that
+ * cell (app/main.f90:240-247).  last (may be NULL) gets the last frame's
+ * header. */
+int sqc_dump_read_frames(sqc_dump_reader *reader, int64_t maxframes, int64_t natoms,
+                         const double cell[9], double *pos, int64_t *nread,
+                         sqc_dump_frame *last);
+int sqc_dump_close(sqc_dump_reader *reader);
+const char *sqc_dump_error_string(int code);
+/* Detail of the error that ended the reader, with its line number. */
+const char *sqc_dump_message(const sqc_dump_reader *reader);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,16 @@ class Info(_c.Structure):
     ]
 
 
+class DumpFrame(_c.Structure):
+    _fields_ = [
+        ("natoms", _c.c_int64), ("timestep", _c.c_int64), ("cell", _c.c_double * 9),
+        ("origin", _c.c_double * 3), ("pbc", _c.c_int32 * 3), ("triclinic", _c.c_int32),
+        ("scaled", _c.c_int32), ("has_id", _c.c_int32), ("unwrapped", _c.c_int32),
+        ("unwrapped_from_dump", _c.c_int32), ("n_ambiguous", _c.c_int64),
+        ("lineno", _c.c_int64), ("frames", _c.c_int64),
+    ]
+
+
 # Every entry point of include/sqc_hip.h with its ctypes signature.
 SIGNATURES = {
     "sqc_hip_device_count": (_c.c_int, [_c.POINTER(_c.c_int)]),
@@ -66,6 +76,14 @@ SIGNATURES = {
     "sqc_hip_error_string": (_c.c_char_p, [_c.c_int]),
     "sqc_hip_last_error": (_c.c_char_p, []),
     "sqc_hip_config_size": (_c.c_size_t, []),
+    "sqc_dump_open": (_c.c_int, [_c.POINTER(_vp), _c.c_char_p, _c.c_int]),
+    "sqc_dump_next": (_c.c_int, [_vp, _c.POINTER(DumpFrame)]),
+    "sqc_dump_copy": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "sqc_dump_read_frames": (_c.c_int, [_vp, _c.c_int64, _c.c_int64, _vp, _vp,
+                                        _c.POINTER(_c.c_int64), _c.POINTER(DumpFrame)]),
+    "sqc_dump_close": (_c.c_int, [_vp]),
+    "sqc_dump_error_string": (_c.c_char_p, [_c.c_int]),
+    "sqc_dump_message": (_c.c_char_p, [_vp]),
 }
 
 _LIB = None

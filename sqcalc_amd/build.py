@@ -31,6 +31,7 @@ def build_library(verbose=False, jobs=2):
         ("sqc_kernels.hip", ["-x", "hip", "--offload-arch=" + ARCH] + DEVICE_FLAGS),
         ("sqc_host.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off"]
          + DEVICE_FLAGS),
+        ("sqc_dump.cpp", ["-x", "c++", "-ffp-contract=off", "-pthread"]),
     ]
     procs = []
     for src, extra in units:
@@ -43,7 +44,7 @@ def build_library(verbose=False, jobs=2):
         if pr.wait() != 0:
             raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    _run([hipcc, "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)

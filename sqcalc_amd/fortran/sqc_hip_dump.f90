@@ -1,20 +1,37 @@
-!> LAMMPS text dump reader for the sqcalc_hip command.
+!> LAMMPS dump frames for the sqcalc_hip command, through the native reader
+!! of libsqc_hip (sqc_dump_* in include/sqc_hip.h).
 !!
-!! Reads the frames the reference reader accepts for `sqcalc static`
-!! (src/sqc_dump.f90): ITEM: TIMESTEP / NUMBER OF ATOMS / BOX BOUNDS (with
-!! or without the xy xz yz tilt line) / ATOMS with an id, a type and one of
-!! the coordinate triples x y z, xu yu zu (Cartesian) or xs ys zs, xsu ysu zsu
-!! (scaled).  The cell is the restricted triclinic cell of the bounds
-!! (src/sqc_cell.f90:39-63); positions come back Cartesian, in file order.
-!! Status: 0 a frame was read, 1 end of file before a frame, >1 malformed.
+!! The reader follows the reference's lammps_dump_reader_t (src/sqc_dump.f90):
+!! the same header records, the restricted triclinic cell of the bounds
+!! (src/sqc_cell.f90:39-63), positions unwrapped across frames unless the dump
+!! has xu yu zu, and its ierr values as status codes (0 a frame was read, 1 end
+!! of file, 2..11 its errors).  Statuses 12 and 13 are the driver's checks of
+!! app/main.f90:240-247 (atom count, box), raised by read_frames.  The scaled
+!! columns xs ys zs and xsu ysu zsu are read as well.
 module sqc_hip_dump
-   use, intrinsic :: iso_fortran_env, only: real64, int64, iostat_end
+   use, intrinsic :: iso_c_binding
+   use, intrinsic :: iso_fortran_env, only: real64, int64
    implicit none
    private
-   public :: dump_reader_t, dump_frame_t, cell_from_bounds
+   public :: dump_reader_t, dump_frame_t
 
    integer, parameter :: rk = real64, ik = int64
-   integer, parameter :: line_len = 1024, max_cols = 64
+
+   type, bind(C) :: c_dump_frame
+      integer(c_int64_t) :: natoms = 0
+      integer(c_int64_t) :: timestep = 0
+      real(c_double) :: cell(9) = 0.0_c_double
+      real(c_double) :: origin(3) = 0.0_c_double
+      integer(c_int32_t) :: pbc(3) = 0
+      integer(c_int32_t) :: triclinic = 0
+      integer(c_int32_t) :: scaled = 0
+      integer(c_int32_t) :: has_id = 0
+      integer(c_int32_t) :: unwrapped = 0
+      integer(c_int32_t) :: unwrapped_from_dump = 0
+      integer(c_int64_t) :: n_ambiguous = 0
+      integer(c_int64_t) :: lineno = 0
+      integer(c_int64_t) :: frames = 0
+   end type c_dump_frame
 
    type :: dump_frame_t
       integer(ik) :: natoms = 0
@@ -23,221 +40,146 @@ module sqc_hip_dump
       real(rk) :: origin(3) = 0.0_rk
       integer(ik), allocatable :: id(:)
       integer(ik), allocatable :: type_id(:)
-      real(rk), allocatable :: pos(:, :)  ! (3, natoms) Cartesian
+      real(rk), allocatable :: pos(:, :)  ! (3, natoms) Cartesian, continuous
    end type dump_frame_t
 
    type :: dump_reader_t
-      integer :: unit = -1
-      integer :: lineno = 0
+      type(c_ptr) :: handle = c_null_ptr
+      integer(ik) :: lineno = 0
    contains
       procedure :: open => reader_open
       procedure :: next_frame => reader_next
+      procedure :: read_frames => reader_read_frames
+      procedure :: message => reader_message
       procedure :: close => reader_close
    end type dump_reader_t
 
+   interface
+      integer(c_int) function sqc_dump_open(reader, path, nthreads) bind(C, name='sqc_dump_open')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), intent(out) :: reader
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: nthreads
+      end function sqc_dump_open
+
+      integer(c_int) function sqc_dump_next(reader, frame) bind(C, name='sqc_dump_next')
+         import :: c_int, c_ptr, c_dump_frame
+         type(c_ptr), value :: reader
+         type(c_dump_frame), intent(out) :: frame
+      end function sqc_dump_next
+
+      integer(c_int) function sqc_dump_copy(reader, pos, id, type_id) bind(C, name='sqc_dump_copy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: reader, pos, id, type_id
+      end function sqc_dump_copy
+
+      integer(c_int) function sqc_dump_read_frames(reader, maxframes, natoms, cell, pos, nread, &
+                                                   last) bind(C, name='sqc_dump_read_frames')
+         import :: c_int, c_ptr, c_int64_t, c_double, c_dump_frame
+         type(c_ptr), value :: reader
+         integer(c_int64_t), value :: maxframes, natoms
+         real(c_double), intent(in) :: cell(9)
+         real(c_double) :: pos(*)
+         integer(c_int64_t), intent(out) :: nread
+         type(c_dump_frame), intent(out) :: last
+      end function sqc_dump_read_frames
+
+      integer(c_int) function sqc_dump_close(reader) bind(C, name='sqc_dump_close')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: reader
+      end function sqc_dump_close
+
+      type(c_ptr) function sqc_dump_message(reader) bind(C, name='sqc_dump_message')
+         import :: c_ptr
+         type(c_ptr), value :: reader
+      end function sqc_dump_message
+
+      integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: s
+      end function c_strlen
+   end interface
+
 contains
 
-   subroutine reader_open(self, path, ierr)
+   !> Open a dump; nthreads <= 0 lets the library pick (OMP_NUM_THREADS).
+   subroutine reader_open(self, path, ierr, nthreads)
       class(dump_reader_t), intent(inout) :: self
       character(len=*), intent(in) :: path
       integer, intent(out) :: ierr
-      open (newunit=self%unit, file=path, status='old', action='read', iostat=ierr)
-      if (ierr /= 0) self%unit = -1
+      integer, intent(in), optional :: nthreads
+      integer(c_int) :: nt
+      nt = 0
+      if (present(nthreads)) nt = int(nthreads, c_int)
+      ierr = int(sqc_dump_open(self%handle, trim(path)//c_null_char, nt))
       self%lineno = 0
    end subroutine reader_open
 
    subroutine reader_close(self)
       class(dump_reader_t), intent(inout) :: self
-      if (self%unit >= 0) close (self%unit)
-      self%unit = -1
+      integer(c_int) :: rc
+      if (c_associated(self%handle)) rc = sqc_dump_close(self%handle)
+      self%handle = c_null_ptr
    end subroutine reader_close
 
-   subroutine get_line(self, line, ierr)
-      class(dump_reader_t), intent(inout) :: self
-      character(len=*), intent(out) :: line
-      integer, intent(out) :: ierr
-      read (self%unit, '(a)', iostat=ierr) line
-      if (ierr == 0) self%lineno = self%lineno + 1
-   end subroutine get_line
-
-   !> Restricted triclinic cell from LAMMPS bounds (src/sqc_cell.f90:39-63).
-   subroutine cell_from_bounds(lo, hi, tilt, cell, origin)
-      real(rk), intent(in) :: lo(3), hi(3), tilt(3)
-      real(rk), intent(out) :: cell(3, 3), origin(3)
-      real(rk) :: xy, xz, yz, xlo, xhi, ylo, yhi
-      xy = tilt(1)
-      xz = tilt(2)
-      yz = tilt(3)
-      xlo = lo(1) - min(0.0_rk, xy, xz, xy + xz)
-      xhi = hi(1) - max(0.0_rk, xy, xz, xy + xz)
-      ylo = lo(2) - min(0.0_rk, yz)
-      yhi = hi(2) - max(0.0_rk, yz)
-      cell = 0.0_rk
-      cell(:, 1) = [xhi - xlo, 0.0_rk, 0.0_rk]
-      cell(:, 2) = [xy, yhi - ylo, 0.0_rk]
-      cell(:, 3) = [xz, yz, hi(3) - lo(3)]
-      origin = [xlo, ylo, lo(3)]
-   end subroutine cell_from_bounds
-
+   !> The next frame with its atom arrays.
    subroutine reader_next(self, fr, ierr)
       class(dump_reader_t), intent(inout) :: self
-      type(dump_frame_t), intent(inout) :: fr
+      type(dump_frame_t), intent(inout), target :: fr
       integer, intent(out) :: ierr
-      character(len=line_len) :: line
-      character(len=32) :: names(max_cols)
-      real(rk) :: lo(3), hi(3), tilt(3), vals(max_cols), s(3)
-      integer :: i, ncol, cid, ctype, cx, kind, ios
-      integer(ik) :: j, n
-      logical :: triclinic, have_step, have_n, have_box
-
-      have_step = .false.
-      have_n = .false.
-      have_box = .false.
-      n = 0
-      do
-         call get_line(self, line, ios)
-         if (ios == iostat_end) then
-            ierr = 1
-            if (have_step .or. have_n .or. have_box) ierr = 2
-            return
-         end if
-         if (ios /= 0) then
-            ierr = 2
-            return
-         end if
-         line = adjustl(line)
-         if (len_trim(line) == 0) cycle
-         if (index(line, 'ITEM: TIMESTEP') == 1) then
-            call get_line(self, line, ios)
-            read (line, *, iostat=ios) fr%timestep
-            if (ios /= 0) then
-               ierr = 3
-               return
-            end if
-            have_step = .true.
-         else if (index(line, 'ITEM: NUMBER OF ATOMS') == 1) then
-            call get_line(self, line, ios)
-            read (line, *, iostat=ios) n
-            if (ios /= 0 .or. n < 1) then
-               ierr = 4
-               return
-            end if
-            have_n = .true.
-         else if (index(line, 'ITEM: BOX BOUNDS') == 1) then
-            triclinic = index(line, 'xy') > 0
-            tilt = 0.0_rk
-            do i = 1, 3
-               call get_line(self, line, ios)
-               if (triclinic) then
-                  read (line, *, iostat=ios) lo(i), hi(i), tilt(i)
-               else
-                  read (line, *, iostat=ios) lo(i), hi(i)
-               end if
-               if (ios /= 0) then
-                  ierr = 5
-                  return
-               end if
-            end do
-            call cell_from_bounds(lo, hi, tilt, fr%cell, fr%origin)
-            have_box = .true.
-         else if (index(line, 'ITEM: ATOMS') == 1) then
-            if (.not. (have_n .and. have_box)) then
-               ierr = 6
-               return
-            end if
-            call split_words(line(12:), names, ncol)
-            cid = find(names, ncol, ['id  '])
-            ctype = find(names, ncol, ['type'])
-            kind = 0
-            cx = find(names, ncol, ['x   '])
-            if (cx > 0) kind = 1
-            if (cx == 0) then
-               cx = find(names, ncol, ['xu  '])
-               if (cx > 0) kind = 1
-            end if
-            if (cx == 0) then
-               cx = find(names, ncol, ['xs  '])
-               if (cx > 0) kind = 2
-            end if
-            if (cx == 0) then
-               cx = find(names, ncol, ['xsu '])
-               if (cx > 0) kind = 2
-            end if
-            if (cid == 0 .or. ctype == 0 .or. cx == 0) then
-               ierr = 7
-               return
-            end if
-            if (cx + 2 > ncol) then
-               ierr = 7
-               return
-            end if
-            fr%natoms = n
-            if (allocated(fr%pos)) then
-               if (size(fr%pos, 2) /= n) deallocate (fr%pos, fr%id, fr%type_id)
-            end if
-            if (.not. allocated(fr%pos)) allocate (fr%pos(3, n), fr%id(n), fr%type_id(n))
-            do j = 1, n
-               call get_line(self, line, ios)
-               if (ios /= 0) then
-                  ierr = 8
-                  return
-               end if
-               read (line, *, iostat=ios) vals(1:ncol)
-               if (ios /= 0) then
-                  ierr = 9
-                  return
-               end if
-               fr%id(j) = nint(vals(cid), ik)
-               fr%type_id(j) = nint(vals(ctype), ik)
-               if (kind == 1) then
-                  fr%pos(:, j) = vals(cx:cx + 2)
-               else
-                  s = vals(cx:cx + 2)
-                  fr%pos(:, j) = fr%origin + matmul(fr%cell, s)
-               end if
-            end do
-            ierr = 0
-            return
-         end if
-      end do
+      type(c_dump_frame) :: h
+      integer(c_int64_t), allocatable, target :: id(:)
+      integer(c_int32_t), allocatable, target :: ty(:)
+      integer(ik) :: n
+      ierr = int(sqc_dump_next(self%handle, h))
+      self%lineno = h%lineno
+      if (ierr /= 0) return
+      n = h%natoms
+      fr%natoms = n
+      fr%timestep = h%timestep
+      fr%cell = reshape(h%cell, [3, 3])
+      fr%origin = h%origin
+      if (allocated(fr%pos)) then
+         if (size(fr%pos, 2) /= n) deallocate (fr%pos, fr%id, fr%type_id)
+      end if
+      if (.not. allocated(fr%pos)) allocate (fr%pos(3, n), fr%id(n), fr%type_id(n))
+      allocate (id(n), ty(n))
+      ierr = int(sqc_dump_copy(self%handle, c_loc(fr%pos), c_loc(id), c_loc(ty)))
+      fr%id = id
+      fr%type_id = ty
    end subroutine reader_next
 
-   subroutine split_words(text, words, n)
-      character(len=*), intent(in) :: text
-      character(len=*), intent(out) :: words(:)
-      integer, intent(out) :: n
-      integer :: i, start
-      n = 0
-      i = 1
-      do while (i <= len_trim(text))
-         if (text(i:i) == ' ' .or. text(i:i) == char(9)) then
-            i = i + 1
-            cycle
-         end if
-         start = i
-         do while (i <= len_trim(text))
-            if (text(i:i) == ' ' .or. text(i:i) == char(9)) exit
-            i = i + 1
-         end do
-         if (n < size(words)) then
-            n = n + 1
-            words(n) = text(start:i - 1)
-         end if
-      end do
-   end subroutine split_words
+   !> Up to size(pos, 3) frames of natoms atoms straight into pos(3, natoms, :),
+   !! each checked against natoms and cell; nread < size(pos, 3) at the end.
+   subroutine reader_read_frames(self, pos, natoms, cell, nread, ierr)
+      class(dump_reader_t), intent(inout) :: self
+      real(c_double), intent(inout), contiguous :: pos(:, :, :)
+      integer(ik), intent(in) :: natoms
+      real(rk), intent(in) :: cell(3, 3)
+      integer(ik), intent(out) :: nread
+      integer, intent(out) :: ierr
+      type(c_dump_frame) :: h
+      integer(c_int64_t) :: got
+      ierr = int(sqc_dump_read_frames(self%handle, int(size(pos, 3), c_int64_t), &
+                                      int(natoms, c_int64_t), reshape(cell, [9]), pos, got, h))
+      nread = got
+      if (h%lineno > 0) self%lineno = h%lineno
+   end subroutine reader_read_frames
 
-   integer function find(words, n, key)
-      character(len=*), intent(in) :: words(:)
-      integer, intent(in) :: n
-      character(len=*), intent(in) :: key(1)
-      integer :: i
-      find = 0
+   !> Detail of the error that ended the reader, with its line.
+   function reader_message(self) result(text)
+      class(dump_reader_t), intent(in) :: self
+      character(len=:), allocatable :: text
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer :: i, n
+      p = sqc_dump_message(self%handle)
+      n = int(c_strlen(p))
+      call c_f_pointer(p, chars, [n])
+      allocate (character(len=n) :: text)
       do i = 1, n
-         if (trim(words(i)) == trim(key(1))) then
-            find = i
-            return
-         end if
+         text(i:i) = chars(i)
       end do
-   end function find
+   end function reader_message
 
 end module sqc_hip_dump

@@ -39,8 +39,9 @@ program sqcalc_hip
    type(c_ptr) :: plan
    integer(c_int) :: rc
    integer :: ierr, nspec, a, b, unit_out, s
-   integer(ik) :: natoms, j, nb, nframes
-   integer(ik), allocatable :: spec_type(:), ref_id(:), counts(:)
+   integer(ik) :: natoms, j, nb, nframes, got
+   logical :: more
+   integer(ik), allocatable :: spec_type(:), counts(:)
    integer(c_int32_t), allocatable, target :: species_of(:)
    real(c_double), allocatable, target :: amp(:), xcoef(:, :), pos(:, :, :)
    real(c_double), allocatable, target :: num(:), part(:, :, :), den(:), gnum(:)
@@ -71,7 +72,6 @@ program sqcalc_hip
    end if
    natoms = frame%natoms
    ref_cell = frame%cell
-   ref_id = frame%id
 
    ! --- species: the type ids present, ascending (src/sqc_structure_factor.f90:1127-1144)
    call classify()
@@ -108,38 +108,19 @@ program sqcalc_hip
    rc = sqc_hip_plan_info(plan, info)
    if (.not. quiet) call report()
 
-   ! --- frames ------------------------------------------------------------------
+   ! --- frames: the first one, then whole batches straight from the reader ---
    allocate (pos(3, natoms, batch))
-   nb = 0
+   pos(:, :, 1) = frame%pos
+   nb = 1
    nframes = 0
    do
-      if (nframes > 0 .or. nb > 0) then
-         call reader%next_frame(frame, ierr)
-         if (ierr == 1) exit
-         if (ierr /= 0) then
-            write (error_unit, '(a,i0,a,i0,a)') 'sqcalc_hip: malformed dump near line ', &
-               reader%lineno, ' (reader error ', ierr, ')'
-            stop 7
-         end if
-         if (frame%natoms /= natoms) then
-            write (error_unit, '(a)') 'sqcalc_hip: number of atoms changes between frames'
-            stop 8
-         end if
-         if (maxval(abs(frame%cell - ref_cell)) > 1.0e-6_rk*max(1.0_rk, maxval(abs(ref_cell)))) then
-            write (error_unit, '(a)') 'sqcalc_hip: the simulation box changes between frames'
-            stop 9
-         end if
-         if (any(frame%id /= ref_id)) then
-            write (error_unit, '(a)') 'sqcalc_hip: the atom order (id) changes between frames; '// &
-               'sort the dump (for example dump_modify ... sort id) or keep the order fixed'
-            stop 24
-         end if
-      end if
-      nb = nb + 1
-      pos(:, :, nb) = frame%pos
-      if (nb == batch) call flush_batch()
+      call reader%read_frames(pos(:, :, nb + 1:batch), natoms, ref_cell, got, ierr)
+      if (ierr /= 0) call reader_failure(ierr)
+      nb = nb + got
+      more = nb == batch
+      call flush_batch()
+      if (.not. more) exit
    end do
-   call flush_batch()
    call reader%close()
 
    ! --- results -----------------------------------------------------------------
@@ -172,6 +153,30 @@ program sqcalc_hip
    rc = sqc_hip_destroy(plan)
 
 contains
+
+   !> Exit codes of the reference driver for reader failures (app/main.f90:225-247).
+   subroutine reader_failure(code)
+      integer, intent(in) :: code
+      select case (code)
+      case (10)
+         write (error_unit, '(a)') 'sqcalc_hip: the atom order (id) changes between frames; '// &
+            'sort the dump (for example dump_modify ... sort id) or keep the order fixed'
+         stop 24
+      case (11)
+         write (error_unit, '(a)') 'sqcalc_hip: the number of atoms changes between frames'
+         stop 25
+      case (12)
+         write (error_unit, '(a)') 'sqcalc_hip: number of atoms changes between frames'
+         stop 8
+      case (13)
+         write (error_unit, '(a)') 'sqcalc_hip: the simulation box changes between frames'
+         stop 9
+      case default
+         write (error_unit, '(a,i0,a,i0,a)') 'sqcalc_hip: malformed dump near line ', &
+            reader%lineno, ' (reader error ', code, '): '//reader%message()
+         stop 7
+      end select
+   end subroutine reader_failure
 
    subroutine flush_batch()
       if (nb == 0) return

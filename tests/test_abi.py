@@ -14,14 +14,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     text = open(os.path.join(ROOT, "include", "sqc_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sqc_hip_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sqc_(?:hip|dump)_[a-z_]+)\s*\(", text)))
 
 
 def test_library_exports_header():
     from sqcalc_amd import _lib
     lib = _lib.load()
     names = header_functions()
-    assert len(names) >= 16
+    assert len(names) >= 23
+    assert "sqc_dump_read_frames" in names
     for n in names:
         assert hasattr(lib, n), n
         assert n in _lib.SIGNATURES, n

@@ -150,9 +150,12 @@ def test_partials_match_python_host(exe, hip_device, tmp_path, precision, fz):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("change,code", [("natoms", 8), ("box", 9), ("order", 24),
+@pytest.mark.parametrize("change,code", [("natoms", 25), ("box", 9), ("order", 24),
                                          ("truncated", 7)])
 def test_frame_consistency_errors(exe, hip_device, tmp_path, change, code):
+    """Exit codes of the reference driver (app/main.f90:225-247).  A changed
+    atom count in an x y z dump is caught by the reader's unwrapping state
+    first (ierr 11, exit 25), as in the reference."""
     d = load_golden("gas_unit_self")
     frames, types = d["frames"], d["types"]
     p = tmp_path / "bad.dump"
