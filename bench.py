@@ -13,7 +13,9 @@ sums) in frames/s.  Next to it the line reports the plain FP64 (sigma = 2)
 forward, the forward + adjoint of the S(q)-matching loss on the same
 geometry, per-stage times from HIP events, the HBM roofline of the dominant
 kernel and the CPU restatement of the reference FINUFFT path timed on the
-host cores.
+host cores.  An "ingest" object times frames read from a LAMMPS dump of the
+same workload: the native reader alone, dump -> S(q) end to end, and the
+reference's own reader on the same file.
 
   python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--frames-per-step 10]
 """
@@ -45,6 +47,9 @@ def main():
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the batched-trajectory leg (L=80, N=32,000, 1000 frames)")
     ap.add_argument("--c3-frames", type=int, default=1000)
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip the dump-reader and dump-to-S(q) measurement")
+    ap.add_argument("--ingest-frames", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run an elementwise copy of 1 GiB (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -180,6 +185,11 @@ def main():
     if not args.no_c3:
         c3 = _batched_trajectory(args.c3_frames, dev)
 
+    # ---- frames from a LAMMPS dump: native reader, and dump -> S(q) end to end ----
+    ingest = None
+    if not args.no_ingest:
+        ingest = _ingest(wl, host, args.ingest_frames, make, not args.no_cpu)
+
     # ---- CPU restatement of the reference FINUFFT path ----
     cpu = None
     if not args.no_cpu:
@@ -215,6 +225,7 @@ def main():
         "fwd_adj_frames_per_s": round(adj, 2) if adj else None,
         "batched_trajectory": c3,
         "cpu_baseline": cpu,
+        "ingest": ingest,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -276,6 +287,71 @@ def _committed_traffic(kernel):
             return json.load(fh).get(kernel)
     except (OSError, ValueError):
         return None
+
+
+def _ingest(wl, host, nframes, make, with_reference):
+    """A LAMMPS dump of the workload (id type x y z, %.10f, nframes frames of
+    the jittered lattice, wrapped): the native reader's rate into a resident
+    host batch, the rate of dump -> FP32 S(q) through sqcalc_amd.dump.feed
+    (parse of batch k + 1 beside the device work of batch k), and, in the
+    CPU-baseline leg, the reference's own reader (src/sqc_dump.f90 built from
+    its sources) on the same file."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from sqcalc_amd.dump import DumpReader, feed
+    n = wl.natoms
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    L = np.diag(wl.cell)
+    row = "%d %d %.10f %.10f %.10f\n"
+    tmp = tempfile.mkdtemp(prefix="sqc_ingest_")
+    path = os.path.join(tmp, "c2.dump")
+    with open(path, "w") as fh:
+        ids = np.arange(1, n + 1)
+        for f in range(nframes):
+            p = np.mod(host[f % len(host)], L)
+            fh.write("ITEM: TIMESTEP\n%d\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n"
+                     % (f, n))
+            fh.write("".join("%.10f %.10f\n" % (wl.origin[d], wl.origin[d] + L[d])
+                             for d in range(3)))
+            fh.write("ITEM: ATOMS id type x y z\n")
+            cols = np.empty((n, 5), dtype=object)
+            cols[:, 0], cols[:, 1], cols[:, 2:] = ids, wl.types, p
+            fh.write((row * n) % tuple(cols.ravel()))
+    size = os.path.getsize(path)
+    buf = np.ones((nframes, n, 3))
+    best = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        with DumpReader(path, threads) as rd:
+            got, _ = rd.read_frames(buf, n)
+        best = min(best, time.perf_counter() - t0)
+    assert got == nframes
+    sf = make("single", max_batch=8, tune=True)
+    sf.accumulate(buf[:1])                    # tuner and graph capture outside the timing
+    sf.reset()
+    t0 = time.perf_counter()
+    done = feed(sf, path, batch=8, nthreads=threads)
+    e2e = done / (time.perf_counter() - t0)
+    sf.close()
+    out = {"sample": "%d frames of %d atoms, id type x y z %%.10f, %.1f MB" % (nframes, n, size / 1e6),
+           "threads": threads,
+           "reader_frames_per_s": round(nframes / best, 2),
+           "reader_MB_per_s": round(size / best / 1e6, 1),
+           "dump_to_sq_frames_per_s": round(e2e, 2)}
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+    if with_reference and os.path.exists(ref):
+        t0 = time.perf_counter()
+        subprocess.run([ref, path, os.path.join(tmp, "ref.bin")], check=True,
+                       capture_output=True)
+        out["reference_reader_frames_per_s"] = round(nframes / (time.perf_counter() - t0), 3)
+        out["reference_reader_cores"] = 1
+    for name in os.listdir(tmp):
+        os.remove(os.path.join(tmp, name))
+    os.rmdir(tmp)
+    return out
 
 
 def _cpu_baseline(wl, frame):

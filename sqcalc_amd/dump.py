@@ -132,6 +132,18 @@ class DumpReader:
             pass
 
 
+def _host_buffer(shape):
+    """Page-locked host memory when torch has a device (uploads then run at
+    link speed), plain memory otherwise."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.empty(shape, dtype=torch.float64, pin_memory=True).numpy()
+    except ImportError:
+        pass
+    return np.empty(shape)
+
+
 def feed(sf, path, cell=None, batch=64, nthreads=0, max_frames=None):
     """Stream the frames of a dump into ``sf.accumulate``, batch by batch.
 
@@ -141,7 +153,7 @@ def feed(sf, path, cell=None, batch=64, nthreads=0, max_frames=None):
     """
     with DumpReader(path, nthreads) as rd:
         natoms = sf.natoms
-        bufs = [np.empty((batch, natoms, 3)), np.empty((batch, natoms, 3))]
+        bufs = [_host_buffer((batch, natoms, 3)), _host_buffer((batch, natoms, 3))]
         result = {}
 
         def parse(k, cell):

@@ -38,6 +38,8 @@ module sqc_hip_dump
       integer(ik) :: timestep = 0
       real(rk) :: cell(3, 3) = 0.0_rk     ! lattice vectors in the columns
       real(rk) :: origin(3) = 0.0_rk
+      logical :: pbc(3) = .true.
+      logical :: unwrapped_from_dump = .false.
       integer(ik), allocatable :: id(:)
       integer(ik), allocatable :: type_id(:)
       real(rk), allocatable :: pos(:, :)  ! (3, natoms) Cartesian, continuous
@@ -139,6 +141,8 @@ contains
       fr%timestep = h%timestep
       fr%cell = reshape(h%cell, [3, 3])
       fr%origin = h%origin
+      fr%pbc = h%pbc /= 0
+      fr%unwrapped_from_dump = h%unwrapped_from_dump /= 0
       if (allocated(fr%pos)) then
          if (size(fr%pos, 2) /= n) deallocate (fr%pos, fr%id, fr%type_id)
       end if

@@ -2,9 +2,11 @@
 
 ``sqcalc_amd.dump.feed`` parses batch k + 1 on a second thread while the
 library runs batch k.  Its accumulators must equal those of the same frames
-handed to ``accumulate`` directly (the device path is deterministic, and the
-batches split the same way), and the S(q) table must match the FP64 oracle on
-the frames the reader returned.
+handed to ``accumulate`` directly, to 1e-12 relative: the fine grids are
+bit-identical, only the order of the FP64 LDS additions into a workgroup's
+shell sums may differ, while a dropped, repeated or stale frame moves them at
+the 1e-1 level.  The S(q) table must match the FP64 oracle on the frames the
+reader returned.
 """
 
 import numpy as np
@@ -59,8 +61,8 @@ def test_feed_matches_accumulate_and_oracle(hip_device, tmp_path, precision):
     direct.accumulate(np.stack([fr.pos for fr in read]))
     a, b = fed.fetch(), direct.fetch()
     assert a[3] == b[3] == nf
-    np.testing.assert_array_equal(a[0], b[0])
-    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-12, atol=1e-12 * np.abs(b[1]).max())
     run = oracle_run(read[0].cell, read[0].origin, read[0].type_id, {1: "Si", 2: "O"}, "neutron",
                      "mean", 0.0, 6.0, 80, [fr.pos for fr in read])
     tol = 1e-9 if precision == "double" else 1e-5
