@@ -73,7 +73,9 @@ typedef struct sqc_hip_config {
   double origin[3];         /* lower corner of the cell */
   int64_t natoms;
   int32_t nspecies;
-  int32_t reserved0;
+  int32_t mode_partials;    /* with partials: also accumulate Re(rho_a rho_b*) per
+                               kept mode (the pair sums behind the --xrd pair
+                               columns, src/sqc_structure_factor.f90:538-588) */
   const int32_t *species_of;  /* [natoms], 0-based species of each atom */
   const double *amp_const;    /* [nspecies] unit/neutron amplitudes */
   const double *xray_coef;    /* [nspecies][9] IT92 a1..a4, b1..b4, c */
@@ -135,6 +137,12 @@ int sqc_hip_accumulate_device(sqc_hip_plan *plan, const double *pos_device, int6
  * (may be NULL), gnum[K] in reference mode order (may be NULL), nframes. */
 int sqc_hip_fetch(sqc_hip_plan *plan, double *num, double *partial, double *gnum,
                   int64_t *nframes);
+
+/* The per-mode pair sums of a plan made with mode_partials: gpart[npairs][K],
+ * pairs (a, b), a <= b, in species order, modes in the order of gnum.  The
+ * per-mode analogue of partial[] (src/sqc_structure_factor.f90:538-588 bins
+ * them by two-theta with the amplitudes and Lorentz-polarization factor). */
+int sqc_hip_fetch_mode_partials(sqc_hip_plan *plan, double *gpart);
 
 /* Zero the accumulators. */
 int sqc_hip_reset(sqc_hip_plan *plan);

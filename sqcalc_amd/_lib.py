@@ -29,7 +29,7 @@ class Config(_c.Structure):
         ("want_grid", _c.c_int32), ("nq", _c.c_int32),
         ("qmin", _c.c_double), ("qmax", _c.c_double), ("eps", _c.c_double),
         ("upsampfac", _c.c_double), ("cell", _c.c_double * 9), ("origin", _c.c_double * 3),
-        ("natoms", _c.c_int64), ("nspecies", _c.c_int32), ("reserved0", _c.c_int32),
+        ("natoms", _c.c_int64), ("nspecies", _c.c_int32), ("mode_partials", _c.c_int32),
         ("species_of", _c.POINTER(_c.c_int32)), ("amp_const", _c.POINTER(_c.c_double)),
         ("xray_coef", _c.POINTER(_c.c_double)), ("tune", _c.c_int32),
         ("max_batch", _c.c_int32),
@@ -66,6 +66,7 @@ SIGNATURES = {
     "sqc_hip_accumulate": (_c.c_int, [_vp, _vp, _c.c_int64]),
     "sqc_hip_accumulate_device": (_c.c_int, [_vp, _vp, _c.c_int64]),
     "sqc_hip_fetch": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "sqc_hip_fetch_mode_partials": (_c.c_int, [_vp, _vp]),
     "sqc_hip_reset": (_c.c_int, [_vp]),
     "sqc_hip_forward_device": (_c.c_int, [_vp, _vp, _c.c_int64, _vp, _vp, _vp]),
     "sqc_hip_adjoint_device": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -404,7 +404,7 @@ void free_plan(sqc_hip_plan *h) {
                   p.d_species, p.d_W1, p.d_W1b, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
                   p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
                   p.d_line_off, p.d_line_d23, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
-                  p.d_gnum, p.d_slots, p.d_gs, p.d_rho, h->g_pos, h->g_out, h->g_gs,
+                  p.d_gnum, p.d_gpart, p.d_slots, p.d_gs, p.d_rho, h->g_pos, h->g_out, h->g_gs,
                   h->g_grad, h->g_rho};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -731,6 +731,11 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if (cfg->want_grid) {
     if ((rc = alloc(&p.d_gnum, sizeof(double) * p.ncand, p.bytes))) return rc;
     HIP_TRY(hipMemset(p.d_gnum, 0, sizeof(double) * p.ncand), "hipMemset");
+  }
+  if (cfg->mode_partials && g.partials && g.ncomp > 1) {
+    const size_t n = sizeof(double) * (size_t)(g.ncomp - 1) * p.ncand;
+    if ((rc = alloc(&p.d_gpart, n, p.bytes))) return rc;
+    HIP_TRY(hipMemset(p.d_gpart, 0, n), "hipMemset");
   }
   if (int e = sqc::kernel_lds_limits(p))
     return hip_fail((hipError_t)e, "setting the LDS size of the kernels");
@@ -1132,6 +1137,61 @@ int sqc_hip_accumulate(sqc_hip_plan *plan, const double *pos, int64_t nframes) {
   return SQC_HIP_OK;
 }
 
+// Candidate slot of every kept mode, in the reference's enumeration order
+// (src/sqc_structure_factor.f90:340-399): a mode outside the canonical half
+// space reads the slot of -h, whose |rho|^2 and Re(rho_a rho_b*) are equal.
+static std::vector<int64_t> mode_candidates(const sqc::Plan &p) {
+  const sqc::Geo &g = p.geo;
+  ModeGeom mg{};
+  std::memcpy(mg.b, g.b, sizeof(mg.b));
+  mg.qmin = p.cfg.qmin;
+  mg.qmax = p.cfg.qmax;
+  mg.nq = p.cfg.nq;
+  mg.dq = p.dq;
+  std::vector<int64_t> out;
+  out.reserve((size_t)p.nmodes);
+  for (int64_t p3 = 1; p3 <= p.m[2]; ++p3) {
+    const int h3 = (int)(p3 - (p.m[2] + 1) / 2);
+    for (int64_t p2 = 1; p2 <= p.m[1]; ++p2) {
+      const int h2 = (int)(p2 - (p.m[1] + 1) / 2);
+      for (int64_t p1 = 1; p1 <= p.m[0]; ++p1) {
+        const int h1 = (int)(p1 - (p.m[0] + 1) / 2);
+        if (h1 == 0 && h2 == 0 && h3 == 0) continue;
+        if (!mg.kept(mg.qlen(h1, h2, h3))) continue;
+        int c1 = h1, c2 = h2, c3 = h3;
+        const bool canon = c3 > 0 || (c3 == 0 && (c2 > 0 || (c2 == 0 && c1 > 0)));
+        if (!canon) {
+          c1 = -c1;
+          c2 = -c2;
+          c3 = -c3;
+        }
+        const int line = p.loff[c3] + (c2 - p.lo2[c3]);
+        out.push_back(p.line_off[line] + (c1 - p.line_info[4 * line + 2]));
+      }
+    }
+  }
+  return out;
+}
+
+int sqc_hip_fetch_mode_partials(sqc_hip_plan *plan, double *gpart) {
+  if (int rc = check_plan(plan)) return rc;
+  sqc::Plan &p = plan->p;
+  if (!gpart) return fail(SQC_HIP_ERR_BAD_ARG, "null gpart");
+  if (!p.d_gpart)
+    return fail(SQC_HIP_ERR_BAD_ARG, "plan was created without mode_partials (and partials)");
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  const int npairs = p.geo.ncomp - 1;
+  std::vector<double> gc((size_t)npairs * p.ncand);
+  HIP_TRY(hipMemcpy(gc.data(), p.d_gpart, sizeof(double) * gc.size(), hipMemcpyDeviceToHost),
+          "download mode partials");
+  const std::vector<int64_t> cand = mode_candidates(p);
+  const size_t K = cand.size();
+  for (int q = 0; q < npairs; ++q)
+    for (size_t k = 0; k < K; ++k) gpart[(size_t)q * K + k] = gc[(size_t)q * p.ncand + cand[k]];
+  return SQC_HIP_OK;
+}
+
 int sqc_hip_fetch(sqc_hip_plan *plan, double *num, double *partial, double *gnum,
                   int64_t *nframes) {
   if (int rc = check_plan(plan)) return rc;
@@ -1162,34 +1222,8 @@ int sqc_hip_fetch(sqc_hip_plan *plan, double *num, double *partial, double *gnum
     std::vector<double> gc(p.ncand);
     HIP_TRY(hipMemcpy(gc.data(), p.d_gnum, sizeof(double) * p.ncand, hipMemcpyDeviceToHost),
             "download grid");
-    ModeGeom mg{};
-    std::memcpy(mg.b, g.b, sizeof(mg.b));
-    mg.qmin = p.cfg.qmin;
-    mg.qmax = p.cfg.qmax;
-    mg.nq = p.cfg.nq;
-    mg.dq = p.dq;
-    int64_t k = 0;
-    for (int64_t p3 = 1; p3 <= p.m[2]; ++p3) {
-      const int h3 = (int)(p3 - (p.m[2] + 1) / 2);
-      for (int64_t p2 = 1; p2 <= p.m[1]; ++p2) {
-        const int h2 = (int)(p2 - (p.m[1] + 1) / 2);
-        for (int64_t p1 = 1; p1 <= p.m[0]; ++p1) {
-          const int h1 = (int)(p1 - (p.m[0] + 1) / 2);
-          if (h1 == 0 && h2 == 0 && h3 == 0) continue;
-          if (!mg.kept(mg.qlen(h1, h2, h3))) continue;
-          int c1 = h1, c2 = h2, c3 = h3;
-          const bool canon = c3 > 0 || (c3 == 0 && (c2 > 0 || (c2 == 0 && c1 > 0)));
-          if (!canon) {
-            c1 = -c1;
-            c2 = -c2;
-            c3 = -c3;
-          }
-          const int line = p.loff[c3] + (c2 - p.lo2[c3]);
-          const int64_t c = p.line_off[line] + (c1 - p.line_info[4 * line + 2]);
-          gnum[k++] = gc[c];
-        }
-      }
-    }
+    const std::vector<int64_t> cand = mode_candidates(p);
+    for (size_t k = 0; k < cand.size(); ++k) gnum[k] = gc[cand[k]];
   }
   if (nframes) *nframes = p.nframes;
   return SQC_HIP_OK;
@@ -1202,6 +1236,8 @@ int sqc_hip_reset(sqc_hip_plan *plan) {
   HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
   HIP_TRY(hipMemset(p.d_acc, 0, sizeof(double) * 2 * p.geo.ncomp * p.geo.nq), "hipMemset");
   if (p.d_gnum) HIP_TRY(hipMemset(p.d_gnum, 0, sizeof(double) * p.ncand), "hipMemset");
+  if (p.d_gpart)
+    HIP_TRY(hipMemset(p.d_gpart, 0, sizeof(double) * (p.geo.ncomp - 1) * p.ncand), "hipMemset");
   p.nframes = 0;
   return SQC_HIP_OK;
 }

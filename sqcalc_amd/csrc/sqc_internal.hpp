@@ -137,6 +137,7 @@ struct Plan {
   double *d_den = nullptr, *d_cnt = nullptr;
   double *d_wgpart = nullptr, *d_acc = nullptr, *d_frame = nullptr;
   double *d_gnum = nullptr;
+  double *d_gpart = nullptr;     // [npairs][ncand] per-mode pair sums (mode_partials)
   double *d_slots = nullptr;
   double *d_gs = nullptr;
   void *d_rho = nullptr;

@@ -1107,7 +1107,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
                                                   const double *__restrict__ line_d23,
                                                   double *__restrict__ wgpart,
                                                   typename C2<T>::t *__restrict__ rho_out,
-                                                  int64_t ncand, double *__restrict__ gnum) {
+                                                  int64_t ncand, double *__restrict__ gnum,
+                                                  double *__restrict__ gpart) {
   using V = typename C2<T>::t;
   const int lb = LB > 0 ? LB : lb_;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1192,6 +1193,16 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
 #pragma unroll
           for (int b = a; b < FM && b < F; ++b, ++pidx)
             atomicAdd(&acc[pidx * nq + s], 2.0 * (rr[a] * rr[b] + ri[a] * ri[b]));
+      }
+      // per-mode pair sums (one thread owns candidate c within a frame), kept
+      // out of the loop above so the plain path compiles as before
+      if (gpart && F > 1) {
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < FM && a < F; ++a)
+#pragma unroll
+          for (int b = a; b < FM && b < F; ++b, ++q)
+            gpart[(size_t)q * ncand + c] += rr[a] * rr[b] + ri[a] * ri[b];
       }
     };
     // one copy of the body: the prefetched index of round k is picked by a
@@ -1854,7 +1865,7 @@ static int forward_x_t(Plan &p, const void *W2, hipStream_t st, int mode, double
               dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups, (const V *)W2,
               (const V *)p.d_tw[0], (const int4 *)p.d_line_info, p.d_line_off, p.d_stab,
               p.d_ph[0], p.d_line_d23, p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
-              mode == 0 ? p.d_gnum : nullptr);
+              mode == 0 ? p.d_gnum : nullptr, mode == 0 ? p.d_gpart : nullptr);
         };
         if (p.lb == 2)
           go(std::integral_constant<int, 2>{});

@@ -47,7 +47,7 @@ module sqc_hip
       real(c_double) :: origin(3) = 0.0_c_double
       integer(c_int64_t) :: natoms = 0
       integer(c_int32_t) :: nspecies = 0
-      integer(c_int32_t) :: reserved0 = 0
+      integer(c_int32_t) :: mode_partials = 0
       type(c_ptr) :: species_of = c_null_ptr
       type(c_ptr) :: amp_const = c_null_ptr
       type(c_ptr) :: xray_coef = c_null_ptr

@@ -44,7 +44,7 @@ class HipStructureFactor:
     def __init__(self, cell, origin, types, mapping=None, weight="unit", norm="mean",
                  qmin=0.0, qmax=20.0, nq=500, eps=None, precision="double", partials=True,
                  faber_ziman=False, want_grid=False, device=0, upsampfac=0.0, max_batch=0,
-                 tune=False):
+                 tune=False, mode_partials=False):
         self.lib = _lib.load()
         self.cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
         self.origin = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
@@ -93,6 +93,8 @@ class HipStructureFactor:
         cfg.norm = _lib.NORM[norm]
         cfg.partials = 1 if self.partials else 0
         cfg.want_grid = 1 if want_grid else 0
+        cfg.mode_partials = 1 if (mode_partials and self.partials) else 0
+        self.mode_partials = bool(cfg.mode_partials)
         cfg.nq = self.nq
         cfg.qmin, cfg.qmax, cfg.eps = self.qmin, self.qmax, float(eps)
         cfg.upsampfac = float(upsampfac)
@@ -286,6 +288,58 @@ class HipStructureFactor:
         frames = float(max(nframes, 1))
         s = np.where(gden > 0, gnum / (frames * np.where(gden > 0, gden, 1.0)), 0.0)
         return qvec, s
+
+    def mode_qlen(self):
+        """|q| of every kept mode, in the reference's mode order."""
+        K = int(self.info.nmodes)
+        qvec = np.zeros((K, 3))
+        hkl = np.zeros((K, 3), dtype=np.int32)
+        _lib.check(self.lib.sqc_hip_modes(self.plan, hkl.ctypes.data, qvec.ctypes.data, None),
+                   "modes")
+        return np.linalg.norm(qvec, axis=1)
+
+    def fetch_mode_partials(self):
+        """Frame sums of Re(rho_a rho_b*) per kept mode, [npairs][K]."""
+        S = len(self.species_type)
+        out = np.zeros((S * (S + 1) // 2, int(self.info.nmodes)))
+        _lib.check(self.lib.sqc_hip_fetch_mode_partials(self.plan, out.ctypes.data),
+                   "fetch_mode_partials")
+        return out
+
+    def _amplitudes(self, qlen):
+        """w_a(|q|) per species and mode (src/sqc_weights.f90:136-154)."""
+        out = np.zeros((len(self.species_type), qlen.size))
+        for i in range(len(self.species_type)):
+            if self.weight == "xray":
+                c = self.xcoef[i]
+                st = (qlen / (4.0 * np.pi)) ** 2
+                out[i] = c[8] + sum(c[k] * np.exp(-c[4 + k] * st) for k in range(4))
+            else:
+                out[i] = self.amp[i]
+        return out
+
+    def xrd(self, lam, tth_min=1.0, tth_max=179.0, step=0.0, lp=True, pbc=(True, True, True)):
+        """Powder pattern of the accumulated frames (needs want_grid; the pair
+        columns need mode_partials).  The plan's q window should be
+        xrd.q_range(lam, tth_min, tth_max), as the reference sets it."""
+        from . import xrd as X
+        _, _, gnum, nframes = self.fetch()
+        if gnum is None:
+            raise ValueError("the XRD pattern needs a plan made with want_grid")
+        qlen = self.mode_qlen()
+        gpart = self.fetch_mode_partials() if self.mode_partials else None
+        pat = X.pattern(gnum, gpart, self._amplitudes(qlen), qlen, nframes, self.natoms, lam,
+                        tth_min, tth_max, step, lp, self.cell, pbc)
+        pat["nframes"] = nframes
+        return pat
+
+    def format_xrd(self, lam, tth_min=1.0, tth_max=179.0, step=0.0, lp=True,
+                   pbc=(True, True, True)):
+        """The --xrd text table (src/sqc_structure_factor.f90:964-1024)."""
+        from . import xrd as X
+        pat = self.xrd(lam, tth_min, tth_max, step, lp, pbc)
+        labels = [lab[2:-1] for lab in self.column_labels()] if pat["pairs"] else []
+        return X.format_pattern(pat, lam, tth_min, tth_max, pat["nframes"], lp, labels)
 
     def _mode_den(self, qlen):
         if self.norm == "n":
