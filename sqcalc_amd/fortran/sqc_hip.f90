@@ -73,7 +73,7 @@ module sqc_hip
    end type sqc_hip_info_t
 
    public :: sqc_hip_device_count, sqc_hip_plan_create, sqc_hip_plan_info, sqc_hip_shells, &
-             sqc_hip_modes, sqc_hip_accumulate, sqc_hip_fetch, sqc_hip_reset, &
+             sqc_hip_modes, sqc_hip_accumulate, sqc_hip_fetch, sqc_hip_fetch_mode_partials, sqc_hip_reset, &
              sqc_hip_gradient, sqc_hip_destroy, sqc_hip_config_size
    public :: sqc_hip_message, sqc_hip_detail, sqc_hip_layout_ok
 
@@ -124,6 +124,12 @@ module sqc_hip
          type(c_ptr), value :: partial, gnum
          integer(c_int64_t), intent(out) :: nframes
       end function sqc_hip_fetch
+
+      integer(c_int) function sqc_hip_fetch_mode_partials(plan, gpart) &
+         bind(C, name='sqc_hip_fetch_mode_partials')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan, gpart
+      end function sqc_hip_fetch_mode_partials
 
       integer(c_int) function sqc_hip_reset(plan) bind(C, name='sqc_hip_reset')
          import :: c_int, c_ptr

@@ -8,7 +8,8 @@
 !!                     [--qmin Q] [--qmax Q] [--nq N] [--eps E]
 !!                     [--precision double|single] [--norm mean|self|n]
 !!                     [--partials|--no-partials] [-fz] [--gpu-id K]
-!!                     [--grid FILE] [--tune] [--quiet] [OUT|-]
+!!                     [--grid FILE] [--xrd FILE --xrd-lambda L [--xrd-range MIN MAX]
+!!                     [--xrd-step S] [--lp|--no-lp]] [--tune] [--quiet] [OUT|-]
 program sqcalc_hip
    use, intrinsic :: iso_c_binding
    use, intrinsic :: iso_fortran_env, only: error_unit, output_unit, real64, int64
@@ -20,6 +21,7 @@ program sqcalc_hip
    integer, parameter :: rk = real64, ik = int64
    integer, parameter :: batch = 8
    character(len=*), parameter :: version = 'sqcalc_hip 1.0 (libsqc_hip ABI 1)'
+   real(rk), parameter :: deg = acos(-1.0_rk)/180.0_rk
 
    ! options (defaults of src/sqc_options.f90)
    character(len=:), allocatable :: input, output, grid_output, mapping_text, msg
@@ -27,6 +29,11 @@ program sqcalc_hip
    real(rk) :: qmin = 0.0_rk, qmax = 20.0_rk, eps = 1.0e-9_rk
    logical :: eps_given = .false., partials = .true., faber_ziman = .false., quiet = .false.
    logical :: tune = .false., want_grid = .false.
+   logical :: qmin_given = .false., qmax_given = .false., weight_given = .false.
+   ! powder XRD (src/sqc_options.f90:90-104, :620-660)
+   character(len=:), allocatable :: xrd_output
+   real(rk) :: xrd_lambda = 0.0_rk, xrd_min = 1.0_rk, xrd_max = 179.0_rk, xrd_step = 0.0_rk
+   logical :: xrd_lp = .true., xrd_opts = .false.
 
    ! mapping -m TYPE:LABEL
    integer(ik), allocatable :: map_type(:)
@@ -83,7 +90,8 @@ program sqcalc_hip
    cfg%weight_kind = int(weight_kind, c_int32_t)
    cfg%norm = int(norm, c_int32_t)
    cfg%partials = merge(1_c_int32_t, 0_c_int32_t, partials)
-   cfg%want_grid = merge(1_c_int32_t, 0_c_int32_t, want_grid)
+   cfg%want_grid = merge(1_c_int32_t, 0_c_int32_t, want_grid .or. allocated(xrd_output))
+   cfg%mode_partials = merge(1_c_int32_t, 0_c_int32_t, partials .and. allocated(xrd_output))
    cfg%nq = int(nq, c_int32_t)
    cfg%qmin = qmin
    cfg%qmax = qmax
@@ -125,7 +133,7 @@ program sqcalc_hip
 
    ! --- results -----------------------------------------------------------------
    allocate (num(nq), part(nq, nspec, nspec), den(nq), shell_count(nq))
-   if (want_grid) then
+   if (want_grid .or. allocated(xrd_output)) then
       allocate (gnum(info%nmodes))
       rc = sqc_hip_fetch(plan, num, c_loc(part), c_loc(gnum), nfr)
    else
@@ -150,6 +158,7 @@ program sqcalc_hip
    call write_table(unit_out)
    if (unit_out /= output_unit) close (unit_out)
    if (want_grid) call write_grid()
+   if (allocated(xrd_output)) call write_xrd()
    rc = sqc_hip_destroy(plan)
 
 contains
@@ -382,6 +391,138 @@ contains
       close (u)
    end subroutine write_grid
 
+   !> Powder pattern (src/sqc_structure_factor.f90:846-904 bins, :964-1024
+   !! table) from the device's per-mode frame sums: gnum for the total and,
+   !! with partials, the per-mode Re(rho_a rho_b*) sums for the pair columns.
+   subroutine write_xrd()
+      real(c_double), allocatable, target :: qvec(:, :), gpart(:, :)
+      integer(c_int32_t), allocatable, target :: hkl(:, :)
+      real(rk), allocatable :: tot(:), pr(:, :), wsp(:)
+      integer, allocatable :: bin(:)
+      real(rk) :: frames, q, tth, st, lp, widths(3), cr(3), vol, bound, theta_max, step, fn
+      integer(ik) :: k
+      integer :: u, nb, i, c, pidx, npair
+      logical :: has_pairs
+      allocate (qvec(3, info%nmodes), hkl(3, info%nmodes))
+      rc = sqc_hip_modes(plan, c_loc(hkl), c_loc(qvec), c_null_ptr)
+      has_pairs = partials
+      npair = nspec*(nspec + 1)/2
+      if (has_pairs .and. nspec > 1) then
+         allocate (gpart(info%nmodes, npair))
+         if (rc == sqc_hip_ok) rc = sqc_hip_fetch_mode_partials(plan, c_loc(gpart))
+      end if
+      if (rc /= sqc_hip_ok) then
+         write (error_unit, '(a)') 'sqcalc_hip: '//sqc_hip_message(rc)//': '//sqc_hip_detail()
+         stop 19
+      end if
+      ! bin width from the narrowest cell width when --xrd-step is not given
+      step = xrd_step
+      if (step <= 0.0_rk) then
+         vol = abs(ref_cell(1, 1)*(ref_cell(2, 2)*ref_cell(3, 3) - ref_cell(3, 2)*ref_cell(2, 3)) &
+                   - ref_cell(1, 2)*(ref_cell(2, 1)*ref_cell(3, 3) - ref_cell(3, 1)*ref_cell(2, 3)) &
+                   + ref_cell(1, 3)*(ref_cell(2, 1)*ref_cell(3, 2) - ref_cell(3, 1)*ref_cell(2, 2)))
+         do i = 1, 3
+            associate (aj => ref_cell(:, mod(i, 3) + 1), ak => ref_cell(:, mod(i + 1, 3) + 1))
+               cr = [aj(2)*ak(3) - aj(3)*ak(2), aj(3)*ak(1) - aj(1)*ak(3), aj(1)*ak(2) - aj(2)*ak(1)]
+            end associate
+            widths(i) = vol/sqrt(sum(cr*cr))
+         end do
+         bound = minval(widths)
+         theta_max = min(0.5_rk*deg*xrd_max, 60.0_rk*deg)
+         step = (xrd_lambda/(bound*cos(theta_max)))/deg
+      end if
+      nb = max(1, ceiling((xrd_max - xrd_min)/step))
+      step = (xrd_max - xrd_min)/real(nb, rk)
+      allocate (tot(nb), pr(nb, npair), bin(info%nmodes), wsp(nspec))
+      tot = 0.0_rk
+      pr = 0.0_rk
+      do k = 1, info%nmodes
+         q = norm2(qvec(:, k))
+         tth = 2.0_rk*asin(max(-1.0_rk, min(1.0_rk, q*xrd_lambda/(4.0_rk*acos(-1.0_rk)))))/deg
+         i = min(max(int((tth - xrd_min)/step) + 1, 1), nb)
+         lp = 1.0_rk
+         if (xrd_lp) lp = (1.0_rk + cos(tth*deg)**2)/(sin(0.5_rk*tth*deg)**2*cos(0.5_rk*tth*deg))
+         tot(i) = tot(i) + gnum(k)*lp
+         if (.not. has_pairs) cycle
+         do c = 1, nspec
+            if (weight_kind == sqc_hip_weight_xray) then
+               st = (q/(4.0_rk*acos(-1.0_rk)))**2
+               wsp(c) = xcoef(9, c) + sum(xcoef(1:4, c)*exp(-xcoef(5:8, c)*st))
+            else
+               wsp(c) = amp(c)
+            end if
+         end do
+         if (nspec == 1) then
+            pr(i, 1) = pr(i, 1) + gnum(k)*lp
+         else
+            pidx = 0
+            do a = 1, nspec
+               do b = a, nspec
+                  pidx = pidx + 1
+                  pr(i, pidx) = pr(i, pidx) + wsp(a)*wsp(b)*gpart(k, pidx)*lp
+               end do
+            end do
+         end if
+      end do
+      open (newunit=u, file=xrd_output, status='replace', action='write', iostat=ierr)
+      if (ierr /= 0) then
+         write (error_unit, '(a)') 'sqcalc_hip: cannot write the XRD file "'//xrd_output//'"'
+         stop 11
+      end if
+      frames = real(max(nfr, 1_c_int64_t), rk)
+      fn = 1.0_rk/(frames*real(natoms, rk))
+      if (nfr == 1) then
+         write (u, '(a,f0.6,a,f0.4,a,f0.4,a,f0.5,a,i0,a)') '# xrd: lambda ', xrd_lambda, &
+            ' A, 2theta ', xrd_min, ' to ', xrd_max, ' deg, step ', step, ' deg, ', nb, &
+            ' bins, 1 frame'
+      else
+         write (u, '(a,f0.6,a,f0.4,a,f0.4,a,f0.5,a,i0,a,i0,a)') '# xrd: lambda ', xrd_lambda, &
+            ' A, 2theta ', xrd_min, ' to ', xrd_max, ' deg, step ', step, ' deg, ', nb, &
+            ' bins, ', nfr, ' frames'
+      end if
+      if (xrd_lp) then
+         write (u, '(a)') '# I(2theta) = sum over the reciprocal lattice points in each '// &
+            'bin of |rho(q)|^2 times the Lorentz-polarization factor, per atom'
+      else
+         write (u, '(a)') '# I(2theta) = sum over the reciprocal lattice points in each '// &
+            'bin of |rho(q)|^2, without the Lorentz-polarization factor, per atom'
+      end if
+      if (has_pairs) write (u, '(a)') '# the pair columns add up to the total: '// &
+         'I = sum_a I(a-a) + 2 sum_{a<b} I(a-b)'
+      write (u, '(a)', advance='no') '# 2theta[deg] I'
+      if (has_pairs) then
+         do a = 1, nspec
+            do b = a, nspec
+               write (u, '(a)', advance='no') ' I('//pair_name(a, b)//')'
+            end do
+         end do
+      end if
+      write (u, '(a)') ''
+      do i = 1, nb
+         write (u, '(f12.4,2x,es20.12)', advance='no') xrd_min + (real(i, rk) - 0.5_rk)*step, &
+            tot(i)*fn
+         if (has_pairs) then
+            do c = 1, npair
+               write (u, '(2x,es20.12)', advance='no') pr(i, c)*fn
+            end do
+         end if
+         write (u, '(a)') ''
+      end do
+      close (u)
+   end subroutine write_xrd
+
+   function pair_name(a, b) result(pair)
+      integer, intent(in) :: a, b
+      character(len=:), allocatable :: pair
+      character(len=18) :: tmp
+      if (len_trim(spec_label(a)) > 0 .and. len_trim(spec_label(b)) > 0) then
+         pair = trim(spec_label(a))//'-'//trim(spec_label(b))
+      else
+         write (tmp, '(i0,a,i0)') spec_type(a), '-', spec_type(b)
+         pair = trim(tmp)
+      end if
+   end function pair_name
+
    !> The scattering tables this program weights with, one line per entry:
    !! `neutron LABEL b` and `xray LABEL a1..a4 b1..b4 c`.
    subroutine print_elements()
@@ -404,6 +545,8 @@ contains
       write (u, '(a)') '         [--qmin Q] [--qmax Q] [--nq N] [--eps E] [--precision double|single]'
       write (u, '(a)') '         [--norm mean|self|n] [--partials|--no-partials] [-fz] [--gpu-id K]'
       write (u, '(a)') '         [--device gpu] [--grid FILE] [--tune] [--quiet] [OUT|-]'
+      write (u, '(a)') '         [--xrd FILE --xrd-lambda L [--xrd-range MIN MAX] [--xrd-step S]'
+      write (u, '(a)') '          [--lp|--no-lp]]'
    end subroutine usage
 
    subroutine fail_usage(text)
@@ -514,8 +657,27 @@ contains
             faber_ziman = .true.
          case ('--tune')
             tune = .true.
+         case ('--lp', '--no-lp')
+            xrd_lp = name == '--lp'
+            xrd_opts = .true.
+         case ('--xrd-range')
+            ! the two limits are separate arguments (src/sqc_options.f90:307-313)
+            if (.not. inline) then
+               i = i + 1
+               if (i > n) call fail_usage('--xrd-range takes two values: MIN2TH MAX2TH')
+               call get_command_argument(i, arg)
+               value = trim(arg)
+            end if
+            read (value, *, iostat=ios) xrd_min
+            i = i + 1
+            if (i > n .or. ios /= 0) call fail_usage('--xrd-range takes two values: MIN2TH MAX2TH')
+            call get_command_argument(i, arg)
+            read (arg, *, iostat=ios) xrd_max
+            if (ios /= 0) call fail_usage('--xrd-range takes two numbers: MIN2TH MAX2TH in degrees')
+            xrd_opts = .true.
          case ('-i', '--input', '-m', '--mapping', '-w', '--weight', '--qmin', '--qmax', '--nq', &
-               '--eps', '--precision', '--device', '--gpu-id', '--norm', '--grid')
+               '--eps', '--precision', '--device', '--gpu-id', '--norm', '--grid', '--xrd', &
+               '--xrd-lambda', '--xrd-step')
             if (.not. inline) then
                i = i + 1
                if (i > n) call fail_usage(name//' needs a value')
@@ -529,6 +691,7 @@ contains
                mapping_text = value
                call parse_mapping(value)
             case ('-w', '--weight')
+               weight_given = .true.
                select case (trim(lower(value)))
                case ('unit', 'none', '1')
                   weight_kind = sqc_hip_weight_unit
@@ -542,9 +705,11 @@ contains
             case ('--qmin')
                read (value, *, iostat=ios) qmin
                if (ios /= 0 .or. qmin < 0.0_rk) call fail_usage('--qmin must be a non-negative number')
+               qmin_given = .true.
             case ('--qmax')
                read (value, *, iostat=ios) qmax
                if (ios /= 0 .or. qmax <= 0.0_rk) call fail_usage('--qmax must be a positive number')
+               qmax_given = .true.
             case ('--nq')
                read (value, *, iostat=ios) nq
                if (ios /= 0 .or. nq < 1) call fail_usage('--nq must be a positive integer')
@@ -586,6 +751,18 @@ contains
             case ('--grid')
                want_grid = .true.
                grid_output = value
+            case ('--xrd')
+               xrd_output = value
+            case ('--xrd-lambda')
+               read (value, *, iostat=ios) xrd_lambda
+               if (ios /= 0 .or. xrd_lambda <= 0.0_rk) &
+                  call fail_usage('--xrd-lambda must be a positive wavelength')
+               xrd_opts = .true.
+            case ('--xrd-step')
+               read (value, *, iostat=ios) xrd_step
+               if (ios /= 0 .or. xrd_step <= 0.0_rk) &
+                  call fail_usage('--xrd-step must be a positive angle in degrees')
+               xrd_opts = .true.
             end select
          case default
             call fail_usage('unknown option "'//name//'"')
@@ -593,6 +770,19 @@ contains
          i = i + 1
       end do
       if (.not. allocated(input)) call fail_usage('missing input dump file (-i)')
+      ! --xrd sets the q window and the default weights (src/sqc_options.f90:620-660)
+      if (allocated(xrd_output)) then
+         if (xrd_lambda <= 0.0_rk) call fail_usage('--xrd needs --xrd-lambda LAMBDA (the incident wavelength)')
+         if (qmin_given .or. qmax_given) call fail_usage('--xrd takes the q range from '// &
+            '--xrd-lambda and --xrd-range; do not pass --qmin or --qmax with it')
+         if (xrd_min <= 0.0_rk .or. xrd_max >= 180.0_rk .or. xrd_max <= xrd_min) &
+            call fail_usage('--xrd-range needs 0 < MIN2TH < MAX2TH < 180 degrees')
+         if (.not. weight_given) weight_kind = sqc_hip_weight_xray
+         qmin = 4.0_rk*acos(-1.0_rk)*sin(0.5_rk*deg*xrd_min)/xrd_lambda
+         qmax = 4.0_rk*acos(-1.0_rk)*sin(0.5_rk*deg*xrd_max)/xrd_lambda
+      else if (xrd_opts) then
+         call fail_usage('--xrd-lambda, --xrd-range, --xrd-step and --lp/--no-lp belong to --xrd FILE')
+      end if
       if (qmax <= qmin) call fail_usage('qmax must be larger than qmin')
       if (weight_kind /= sqc_hip_weight_unit .and. .not. allocated(map_type)) &
          call fail_usage('weighted schemes need an element mapping, e.g. -m 1:Si,2:O')

@@ -327,9 +327,13 @@ class HipStructureFactor:
         if gnum is None:
             raise ValueError("the XRD pattern needs a plan made with want_grid")
         qlen = self.mode_qlen()
-        gpart = self.fetch_mode_partials() if self.mode_partials else None
+        single = len(self.species_type) == 1
+        gpart = self.fetch_mode_partials() if (self.mode_partials and not single) else None
         pat = X.pattern(gnum, gpart, self._amplitudes(qlen), qlen, nframes, self.natoms, lam,
                         tth_min, tth_max, step, lp, self.cell, pbc)
+        if self.mode_partials and single:
+            # one species: its pair column is the total (w_a^2 |rho_a|^2)
+            pat["pairs"] = [pat["intensity"].copy()]
         pat["nframes"] = nframes
         return pat
 

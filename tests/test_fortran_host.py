@@ -62,6 +62,13 @@ def test_element_tables_match_python(exe):
     (["static", "-i", "x.dump", "--qmin", "5", "--qmax", "4"], 2, "larger than qmin"),
     (["static", "-i", "x.dump", "--precision", "half"], 2, "unknown precision"),
     (["static", "-i", "/nonexistent/x.dump"], 4, "cannot open"),
+    # --xrd rules of src/sqc_options.f90:620-660
+    (["static", "-i", "x.dump", "--xrd", "p.txt"], 2, "--xrd-lambda"),
+    (["static", "-i", "x.dump", "--xrd", "p.txt", "--xrd-lambda", "1.54", "--qmax", "5"], 2,
+     "do not pass --qmin or --qmax"),
+    (["static", "-i", "x.dump", "--xrd-lambda", "1.54"], 2, "belong to --xrd FILE"),
+    (["static", "-i", "x.dump", "--xrd", "p.txt", "--xrd-lambda", "1.54", "--xrd-range", "0",
+      "90"], 2, "0 < MIN2TH < MAX2TH < 180"),
 ])
 def test_option_errors(exe, args, code, text):
     r = run(args)
@@ -179,3 +186,41 @@ def test_frame_consistency_errors(exe, hip_device, tmp_path, change, code):
     print("case fortran frame error", change, flush=True)
     r = run(["static", "-i", str(p), "--quiet", str(tmp_path / "o.txt")])
     assert r.returncode == code, (r.returncode, r.stderr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("partials", [True, False])
+def test_xrd_table_matches_python_host(exe, hip_device, tmp_path, partials):
+    """--xrd through the command against HipStructureFactor.format_xrd on the
+    frames the reader returned: same header, bins and columns to 1e-9."""
+    from sqcalc_amd import HipStructureFactor, xrd
+    from sqcalc_amd.dump import DumpReader
+    from helpers import random_case
+    cell, origin, types, frames = random_case(natoms=500, length=14.0, nframes=3, seed=7)
+    p = tmp_path / "x.dump"
+    write_dump(p, frames, types, [0.0] * 3, [14.0] * 3)
+    out = tmp_path / "xrd.txt"
+    args = ["static", "-i", str(p), "-m", "1:Si,2:O", "--xrd", str(out), "--xrd-lambda",
+            "1.541838", "--xrd-range", "12", "125", "--quiet", str(tmp_path / "sq.txt")]
+    if not partials:
+        args.insert(1, "--no-partials")
+    print("case fortran xrd", partials, flush=True)
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    with DumpReader(p, 2) as rd:
+        read = list(rd)
+    qmin, qmax = xrd.q_range(1.541838, 12.0, 125.0)
+    sf = HipStructureFactor(read[0].cell, read[0].origin, read[0].type_id, {1: "Si", 2: "O"},
+                            weight="xray", qmin=qmin, qmax=qmax, partials=partials,
+                            want_grid=True, mode_partials=partials)
+    sf.accumulate(np.stack([fr.pos for fr in read]))
+    want = sf.format_xrd(1.541838, 12.0, 125.0).splitlines()
+    got = out.read_text().splitlines()
+    nhead = 4 if partials else 3
+    assert got[:nhead] == want[:nhead]
+    assert len(got) == len(want)
+    g = np.array([[float(x) for x in l.split()] for l in got[nhead:]])
+    w = np.array([[float(x) for x in l.split()] for l in want[nhead:]])
+    np.testing.assert_allclose(g[:, 0], w[:, 0], atol=1e-9)
+    assert np.abs(g[:, 1:] - w[:, 1:]).max() <= 1e-9 * np.abs(w[:, 1:]).max()
+    sf.close()
