@@ -333,11 +333,11 @@ def _ingest(wl, host, nframes, make, with_reference):
     sf.accumulate(buf[:1])                    # tuner and graph capture outside the timing
     sf.reset()
     t0 = time.perf_counter()
-    done = feed(sf, path, batch=8, nthreads=threads)
+    done = feed(sf, path, batch=8)          # one worker fewer than the host threads
     e2e = done / (time.perf_counter() - t0)
     sf.close()
     out = {"sample": "%d frames of %d atoms, id type x y z %%.10f, %.1f MB" % (nframes, n, size / 1e6),
-           "threads": threads,
+           "threads": threads, "feed_threads": max(1, threads - 1),
            "reader_frames_per_s": round(nframes / best, 2),
            "reader_MB_per_s": round(size / best / 1e6, 1),
            "dump_to_sq_frames_per_s": round(e2e, 2)}

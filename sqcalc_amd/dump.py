@@ -144,13 +144,19 @@ def _host_buffer(shape):
     return np.empty(shape)
 
 
-def feed(sf, path, cell=None, batch=64, nthreads=0, max_frames=None):
+def feed(sf, path, cell=None, batch=8, nthreads=0, max_frames=None):
     """Stream the frames of a dump into ``sf.accumulate``, batch by batch.
 
     Every frame must keep ``sf.natoms`` atoms and the cell (3, 3) of ``cell``,
-    or of the file's first frame when ``cell`` is None.  Returns the number of
-    frames added.
+    or of the file's first frame when ``cell`` is None.  nthreads <= 0 parses
+    with one worker fewer than the host's threads: the thread that drives the
+    device waits by spinning, and a parse worker that shares its core holds
+    up every frame's join.  Returns the number of frames added.
     """
+    if nthreads <= 0:
+        import os
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        nthreads = max(1, cores - 1)
     with DumpReader(path, nthreads) as rd:
         natoms = sf.natoms
         bufs = [_host_buffer((batch, natoms, 3)), _host_buffer((batch, natoms, 3))]
