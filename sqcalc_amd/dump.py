@@ -99,8 +99,16 @@ class DumpReader:
     def read_frames(self, out, natoms, cell=None):
         """Fill out[:k] (an (F, natoms, 3) float64 C-contiguous array or pinned
         tensor) with the next k <= F frames; returns (k, header of the last)."""
-        ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
-        if tuple(out.shape[1:]) != (natoms, 3):
+        if hasattr(out, "data_ptr"):           # torch tensor (e.g. pinned host memory)
+            import torch
+            if out.dtype != torch.float64 or not out.is_contiguous() or out.device.type != "cpu":
+                raise ValueError("out must be a contiguous float64 host tensor")
+            ptr = out.data_ptr()
+        else:
+            if out.dtype != np.float64 or not out.flags.c_contiguous or not out.flags.writeable:
+                raise ValueError("out must be a writeable C-contiguous float64 array")
+            ptr = out.ctypes.data
+        if out.ndim != 3 or tuple(out.shape[1:]) != (natoms, 3):
             raise ValueError("out must be (F, natoms, 3)")
         cbuf = None
         if cell is not None:

@@ -232,3 +232,16 @@ def test_missing_file_and_closed_reader(tmp_path):
     with pytest.raises(DumpError) as exc:
         DumpReader(tmp_path / "nope.dump")
     assert exc.value.code == 2
+
+
+def test_read_frames_rejects_unsafe_buffers(tmp_path):
+    rng = np.random.default_rng(2)
+    a, traj = _trajectory(rng, 50, 2, 10.0, 0.3)
+    p = _write(tmp_path, "s", _dump_text(a, traj, "id type x y z"))
+    with DumpReader(p, 1) as rd:
+        for bad in (np.zeros((2, 50, 3), dtype=np.float32), np.zeros((2, 3, 50)).transpose(0, 2, 1),
+                    np.zeros((2, 51, 3)), np.zeros((100, 3))):
+            with pytest.raises(ValueError):
+                rd.read_frames(bad, 50)
+        n, _ = rd.read_frames(np.zeros((2, 50, 3)), 50)
+        assert n == 2
