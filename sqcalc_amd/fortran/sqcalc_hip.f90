@@ -427,7 +427,22 @@ contains
             end associate
             widths(i) = vol/sqrt(sum(cr*cr))
          end do
-         bound = minval(widths)
+         ! periodic directions only; none periodic: the first frame's extent
+         bound = huge(1.0_rk)
+         do i = 1, 3
+            if (frame%pbc(i)) bound = min(bound, widths(i))
+         end do
+         if (bound >= huge(1.0_rk)) then
+            do i = 1, 3
+               q = maxval(frame%pos(i, :)) - minval(frame%pos(i, :))
+               if (q > 0.0_rk) bound = min(bound, q)
+            end do
+         end if
+         if (bound <= 0.0_rk .or. bound >= huge(1.0_rk)) then
+            write (error_unit, '(a)') 'sqcalc_hip: cannot derive the XRD bin width from a '// &
+               'degenerate box; pass --xrd-step'
+            stop 19
+         end if
          theta_max = min(0.5_rk*deg*xrd_max, 60.0_rk*deg)
          step = (xrd_lambda/(bound*cos(theta_max)))/deg
       end if
