@@ -193,7 +193,7 @@ def main():
     # ---- CPU restatement of the reference FINUFFT path ----
     cpu = None
     if not args.no_cpu:
-        cpu = _cpu_baseline(wl, host[0])
+        cpu = _cpu_baseline(wl, host[0], int(info.nmodes))
 
     line = {
         "metric": "S(q) frames/s (fwd, FP32 spread, FP64 shell sums) at 96k atoms",
@@ -357,9 +357,14 @@ def _ingest(wl, host, nframes, make, with_reference):
     return out
 
 
-def _cpu_baseline(wl, frame):
+def _cpu_baseline(wl, frame, nmodes):
     """One frame of the CPU restatement of the reference FINUFFT path (sigma
-    1.25, ES kernel, eps 1e-9, FP64, scipy pocketfft), all host cores."""
+    1.25, ES kernel, eps 1e-9, FP64, scipy pocketfft), all host cores; and the
+    FP64 direct sum (the reference's --method direct, src/sqc_structure_factor.f90:1398-1515)
+    on every atom and a 22^3 block of modes, scaled linearly to the frame's
+    kept modes (its cost is atoms x modes)."""
+    import numpy as np
+
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0))
     run = O.Run(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight, norm=wl.norm,
@@ -367,8 +372,21 @@ def _cpu_baseline(wl, frame):
     t0 = time.perf_counter()
     O.nufft_frame(run, frame, eps=1e-9, upsampfac=1.25, nthreads=cores)
     dt = time.perf_counter() - t0
+    side = 22
+    h = np.stack(np.meshgrid(*[np.arange(1, side + 1)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    b = O.reciprocal(wl.cell)
+    md = {"hkl": np.ascontiguousarray(h, dtype=np.int32),
+          "qvec": np.ascontiguousarray(h @ b.T), "qlen": np.zeros(len(h))}
+    t1 = time.perf_counter()
+    O.direct_rho(frame, wl.cell, wl.origin, run.species_of, len(run.species_type), md)
+    dd = time.perf_counter() - t1
+    per_frame = dd * nmodes / len(h)
     return {"value": round(1.0 / dt, 4), "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": "1 frame of the same workload (FP64, sigma 1.25, eps 1e-9, %.1f s)" % dt}
+            "sample": "1 frame of the same workload (FP64, sigma 1.25, eps 1e-9, %.1f s)" % dt,
+            "direct_sum": {"frames_per_s": round(1.0 / per_frame, 6), "seconds_per_frame":
+                           round(per_frame, 1),
+                           "sample": "%d atoms x %d of %d kept modes in %.2f s, scaled to all "
+                                     "modes" % (wl.natoms, len(h), nmodes, dd)}}
 
 
 if __name__ == "__main__":
