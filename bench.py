@@ -263,6 +263,19 @@ def _batched_trajectory(nframes, dev):
             out["tile"] = list(info.tile)
             out["fine_grid"] = list(info.fine)
             out["kept_modes"] = int(info.nmodes)
+            # the frame against the fused byte model, and whether its working
+            # set (W1 + W2) fits the 256 MB Infinity Cache: then the model's
+            # "bandwidth" may exceed the HBM peak
+            nb = sf.stage_bytes()
+            fb = sum(nb[k] for k in ("spread_z", "dft_y", "dft_x_bin"))
+            M3 = (info.modes[2] - 1) // 2
+            w1 = info.nfields * (M3 + 1) * info.fine[1] * info.fine[0] * 8
+            w2 = info.nfields * info.nlines * info.fine[0] * 8
+            out["frame_hbm_model_gb"] = round(fb / 1e9, 4)
+            out["model_GBps"] = round(fb * nframes / dt / 1e9, 1)
+            out["model_frac_of_hbm_peak"] = round(fb * nframes / dt / 1e9 / HBM_PEAK_GBS, 4)
+            out["working_set_mb"] = round((w1 + w2) / 1e6, 1)
+            out["fits_infinity_cache_256mb"] = (w1 + w2) < 256e6
         sf.close()
     return out
 
