@@ -20,7 +20,8 @@ c4  L=44, N=8,000, 3 species: dL/dr of sum_s (S_s - T_s)^2 against the
     oracle's analytic gradient (1e-8 FP64, 1e-5 FP32 at eps 1e-6) and against
     central differences of the GPU FP64 forward on 64 coordinates (h = 1e-4 A).
 c5  triclinic 250 A, N = 10^6, 4 species, xray, 10 partials, q_max 4: FP32
-    against FP64 on every column, 1e-6.
+    against FP64 on every column, 1e-6; the FP64 plan against the oracle's
+    direct sum on the 25 shells below q = 0.2 (dq = 0.008 in both), 1e-9.
 Tolerances use the reference suite's metric max |d| / max(|ref|, 1).
 """
 
@@ -247,3 +248,15 @@ def test_c5_single_against_double(hip_device):
     dev = rel_dev(tables["double"][:, 1:], tables["single"][:, 1:])
     print("  fp32 vs fp64 max deviation %.2e" % dev)
     assert dev < 1e-6
+    # the lowest shells at full size against the direct sum (about 2,100 modes)
+    nlow = 25
+    dq = (wl.qmax - wl.qmin) / wl.nq
+    run = O.Run(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight, norm=wl.norm,
+                qmin=wl.qmin, qmax=wl.qmin + nlow * dq, nq=nlow)
+    run.add_frame(frames[0])
+    ref = run.table()
+    full = tables["double"]
+    np.testing.assert_allclose(full[:nlow, 0], ref[:, 0], atol=1e-12)
+    dev = rel_dev(ref[:, 1:], full[:nlow, 1:])
+    print("  fp64 vs oracle on %d shells: %.2e" % (nlow, dev))
+    assert dev < 1e-9
