@@ -189,8 +189,9 @@ def test_frame_consistency_errors(exe, hip_device, tmp_path, change, code):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("partials", [True, False])
-def test_xrd_table_matches_python_host(exe, hip_device, tmp_path, partials):
+@pytest.mark.parametrize("partials,step,lp", [(True, 0.0, True), (False, 0.0, True),
+                                              (True, 0.5, False)])
+def test_xrd_table_matches_python_host(exe, hip_device, tmp_path, partials, step, lp):
     """--xrd through the command against HipStructureFactor.format_xrd on the
     frames the reader returned: same header, bins and columns to 1e-9."""
     from sqcalc_amd import HipStructureFactor, xrd
@@ -204,7 +205,11 @@ def test_xrd_table_matches_python_host(exe, hip_device, tmp_path, partials):
             "1.541838", "--xrd-range", "12", "125", "--quiet", str(tmp_path / "sq.txt")]
     if not partials:
         args.insert(1, "--no-partials")
-    print("case fortran xrd", partials, flush=True)
+    if step > 0:
+        args[1:1] = ["--xrd-step", str(step)]
+    if not lp:
+        args.insert(1, "--no-lp")
+    print("case fortran xrd", partials, step, lp, flush=True)
     r = run(args)
     assert r.returncode == 0, r.stderr
     with DumpReader(p, 2) as rd:
@@ -214,7 +219,7 @@ def test_xrd_table_matches_python_host(exe, hip_device, tmp_path, partials):
                             weight="xray", qmin=qmin, qmax=qmax, partials=partials,
                             want_grid=True, mode_partials=partials)
     sf.accumulate(np.stack([fr.pos for fr in read]))
-    want = sf.format_xrd(1.541838, 12.0, 125.0).splitlines()
+    want = sf.format_xrd(1.541838, 12.0, 125.0, step=step, lp=lp).splitlines()
     got = out.read_text().splitlines()
     nhead = 4 if partials else 3
     assert got[:nhead] == want[:nhead]
