@@ -154,7 +154,7 @@ def main():
         sd.close()
 
     # ---- forward + adjoint (S(q)-matching loss) ----
-    adj = None
+    adj = adj_graph = None
     if not args.no_adjoint:
         from sqcalc_amd.autograd import sq_matching_loss
         # gradient plans run at eps 1e-6 (the forward default 1e-5 leaves the
@@ -178,6 +178,29 @@ def main():
             loss.backward()
         torch.cuda.synchronize()
         adj = nf / (time.perf_counter() - t0)
+        # the same loss step captured once with torch.cuda.graph and replayed
+        # per frame (positions copied into the captured input)
+        static = pos[0].clone().requires_grad_(True)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                static.grad = None
+                sq_matching_loss(static, sa, target).backward()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        static.grad = None
+        with torch.cuda.graph(graph):
+            sq_matching_loss(static, sa, target).backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nf):
+            with torch.no_grad():
+                static.copy_(pos[k])
+            graph.replay()
+        torch.cuda.synchronize()
+        adj_graph = nf / (time.perf_counter() - t0)
         sa.close()
 
     # ---- batched trajectory: 1000 frames through graph replays, autotuned tiles ----
@@ -223,6 +246,7 @@ def main():
         "fp64_plain_frames_per_s": round(fp64, 2) if fp64 else None,
         "fp32_over_fp64": round(frames_per_s / fp64, 2) if fp64 else None,
         "fwd_adj_frames_per_s": round(adj, 2) if adj else None,
+        "fwd_adj_graph_frames_per_s": round(adj_graph, 2) if adj else None,
         "batched_trajectory": c3,
         "cpu_baseline": cpu,
         "ingest": ingest,

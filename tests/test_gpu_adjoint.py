@@ -101,3 +101,58 @@ def test_adjoint_dense_windows(hip_device, precision):
     print("  max |dg| / max |g| = %.2e" % err)
     assert err <= (TOL[precision] if precision == "double" else 1e-4)
     sf.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_loss_step_under_graph_capture(hip_device, precision):
+    """The forward and adjoint entry points order their work on the caller's
+    stream with events and make no host synchronisation, so a whole loss step
+    (forward, loss, backward) can be captured with torch.cuda.graph and
+    replayed per frame.  Each replay, with eager work and a synchronisation
+    in between, must give the eager gradient of its frame: 1e-10 relative in
+    FP64; in FP32 the FP64 shell sums' atomic order can flip single-precision
+    roundings downstream, so the gradient tolerance applies."""
+    import torch
+
+    from sqcalc_amd import HipStructureFactor
+    from sqcalc_amd.autograd import sq_matching_loss
+
+    cell, origin, types, frames = random_case(natoms=800, length=16.0, nframes=4, seed=5,
+                                              tilt=(0.8, 0.0, -0.4))
+    print("case adjoint graph capture", precision, flush=True)
+    eps = 1e-11 if precision == "double" else 1e-6
+    sf = HipStructureFactor(cell, origin, types, {1: "Si", 2: "O"}, weight="neutron",
+                            norm="mean", qmax=4.0, nq=40, precision=precision, partials=False,
+                            eps=eps, tune=True)
+    target = torch.tensor(np.random.default_rng(1).random(40), device=hip_device)
+    pos = torch.tensor(frames, device=hip_device)
+    ref = []
+    for k in range(len(frames)):
+        x = pos[k].clone().requires_grad_(True)
+        sq_matching_loss(x, sf, target).backward()
+        ref.append(x.grad.clone())
+    static = pos[0].clone().requires_grad_(True)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            static.grad = None
+            sq_matching_loss(static, sf, target).backward()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    static.grad = None
+    with torch.cuda.graph(graph):
+        sq_matching_loss(static, sf, target).backward()
+    tol = 1e-10 if precision == "double" else TOL["single"]
+    for rep in range(3):
+        for k in range(len(frames)):
+            with torch.no_grad():
+                static.copy_(pos[k])
+            graph.replay()
+            junk = (pos[k] * 2.0).sum()
+            torch.cuda.synchronize()
+            dev = float((static.grad - ref[k]).abs().max() / ref[k].abs().max())
+            assert dev < tol, (rep, k, dev)
+            assert float(junk) != 0.0
+    sf.close()
