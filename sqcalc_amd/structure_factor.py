@@ -144,6 +144,20 @@ class HipStructureFactor:
         return S * (S + 1) // 2 if self.partials else 0
 
     # -- accumulation (accumulate_frame) -------------------------------------
+    @classmethod
+    def from_frame(cls, frame, mapping=None, **kw):
+        """Plan for the cell, origin and atom types of a dump frame
+        (sqcalc_amd.dump.Frame), as method_setup takes the first frame
+        (src/sqc_structure_factor.f90:172-182)."""
+        return cls(frame.cell, frame.origin, frame.type_id, mapping, **kw)
+
+    def accumulate_dump(self, path, batch=8, nthreads=0, max_frames=None):
+        """Every frame of a LAMMPS dump (sqcalc_amd.dump.feed: the next batch
+        is parsed while the device runs this one); returns the frames added."""
+        from .dump import feed
+        return feed(self, path, cell=self.cell, batch=batch, nthreads=nthreads,
+                    max_frames=max_frames)
+
     def accumulate(self, positions):
         """Add frames; positions (N, 3) or (F, N, 3) Cartesian, host memory."""
         pos = np.ascontiguousarray(positions, dtype=np.float64)

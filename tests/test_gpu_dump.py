@@ -69,3 +69,13 @@ def test_feed_matches_accumulate_and_oracle(hip_device, tmp_path, precision):
     assert rel_dev(run.table(), fed.table()) < tol
     fed.close()
     direct.close()
+    # the plan from the first frame, the whole file through accumulate_dump
+    from sqcalc_amd import HipStructureFactor as H
+    sf = H.from_frame(read[0], {1: "Si", 2: "O"}, weight="neutron", norm="mean", qmax=6.0,
+                      nq=80, precision=precision, eps=1e-10 if precision == "double" else None,
+                      max_batch=4)
+    assert sf.accumulate_dump(path, batch=3) == nf
+    c = sf.fetch()
+    assert c[3] == nf
+    np.testing.assert_allclose(c[0], b[0], rtol=1e-12, atol=0)
+    sf.close()
