@@ -1,8 +1,9 @@
 """FP64 CPU oracle for the reciprocal-space S(q) path (Python side).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
-CPU-baseline leg of ``bench.py`` import this module, and only as the checker
-or the timed CPU restatement; the product (``sqcalc_amd``) never does.
+CPU-baseline leg of ``bench.py`` import this module, and only as the reference
+results to compare with or the timed CPU restatement; the product
+(``sqcalc_amd``) never does.
 
 It wraps ``libsqc_oracle.so`` (``sqc_oracle.c``, ``sqc_nufft_cpu.c``) and
 restates the host-side bookkeeping of the reference in numpy:
