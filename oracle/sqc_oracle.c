@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (sqcalc_amd/, include/,
  * the Fortran host) links, loads or calls this file; only tests/, the smoke
- * check and the CPU-baseline leg of bench.py do, and only as the checker.
+ * check and the CPU-baseline leg of bench.py do, and only as the reference
+ * results to compare with (or, in bench.py, as the timed CPU path).
  *
  * It restates, in plain C, the reference's exact evaluation of the density
  * amplitudes and of the shell bookkeeping around them:
