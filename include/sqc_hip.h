@@ -242,7 +242,9 @@ typedef struct sqc_dump_frame {
   int64_t frames;            /* frames read so far */
 } sqc_dump_frame;
 
-/* nthreads <= 0: OMP_NUM_THREADS, else the hardware thread count. */
+/* nthreads > 0: that many parse threads; nthreads <= 0: the host's threads
+ * (OMP_NUM_THREADS when set, else the hardware count) less -nthreads, so -1
+ * leaves a core to a thread that drives the device. */
 int sqc_dump_open(sqc_dump_reader **reader, const char *path, int nthreads);
 /* Next frame into the reader's arrays; SQC_DUMP_END at the end.  After an
  * error the reader returns that error from then on. */
@@ -258,6 +260,15 @@ int sqc_dump_copy(const sqc_dump_reader *reader, double *pos, int64_t *id, int32
 int sqc_dump_read_frames(sqc_dump_reader *reader, int64_t maxframes, int64_t natoms,
                          const double cell[9], double *pos, int64_t *nread,
                          sqc_dump_frame *last);
+/* sqc_dump_read_frames on a thread of its own: _begin returns at once, the
+ * batch is written into pos while the caller works (for instance runs the
+ * previous batch through sqc_hip_accumulate); _end waits for it and returns
+ * its status and frame count.  One batch in flight per reader; the reader
+ * must not be used otherwise until _end.  No counterpart in the reference,
+ * whose reader is synchronous. */
+int sqc_dump_read_frames_begin(sqc_dump_reader *reader, int64_t maxframes, int64_t natoms,
+                               const double cell[9], double *pos);
+int sqc_dump_read_frames_end(sqc_dump_reader *reader, int64_t *nread);
 int sqc_dump_close(sqc_dump_reader *reader);
 const char *sqc_dump_error_string(int code);
 /* Detail of the error that ended the reader, with its line number. */

@@ -82,6 +82,8 @@ SIGNATURES = {
     "sqc_dump_copy": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "sqc_dump_read_frames": (_c.c_int, [_vp, _c.c_int64, _c.c_int64, _vp, _vp,
                                         _c.POINTER(_c.c_int64), _c.POINTER(DumpFrame)]),
+    "sqc_dump_read_frames_begin": (_c.c_int, [_vp, _c.c_int64, _c.c_int64, _vp, _vp]),
+    "sqc_dump_read_frames_end": (_c.c_int, [_vp, _c.POINTER(_c.c_int64)]),
     "sqc_dump_close": (_c.c_int, [_vp]),
     "sqc_dump_error_string": (_c.c_char_p, [_c.c_int]),
     "sqc_dump_message": (_c.c_char_p, [_vp]),

@@ -34,6 +34,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cerrno>
 #include <cmath>
@@ -381,6 +382,10 @@ struct sqc_dump_reader {
   std::vector<int32_t> type;
   size_t block_bytes = 0;  // bytes of the last atom block, the next window's guess
   int64_t frames = 0;
+  // a batch read running on a thread of its own (sqc_dump_read_frames_begin)
+  std::thread bg;
+  int bg_rc = 0;
+  int64_t bg_n = 0;
 };
 
 namespace {
@@ -774,8 +779,11 @@ int sqc_dump_open(sqc_dump_reader **reader, const char *path, int nthreads) {
   }
   int n = nthreads;
   if (n <= 0) {
-    n = (int)std::thread::hardware_concurrency();
-    if (const char *env = std::getenv("OMP_NUM_THREADS")) n = std::atoi(env);
+    // all host threads (OMP_NUM_THREADS when set), less -nthreads of them
+    int all = (int)std::thread::hardware_concurrency();
+    if (const char *env = std::getenv("OMP_NUM_THREADS"))
+      if (std::atoi(env) > 0) all = std::atoi(env);
+    n = all + nthreads;
   }
   r->pool = new Pool(std::max(1, std::min(n, 256)));
   *reader = r;
@@ -825,8 +833,33 @@ int sqc_dump_read_frames(sqc_dump_reader *reader, int64_t maxframes, int64_t nat
   return SQC_DUMP_OK;
 }
 
+int sqc_dump_read_frames_begin(sqc_dump_reader *reader, int64_t maxframes, int64_t natoms,
+                               const double cell[9], double *pos) {
+  if (!reader || maxframes < 0 || natoms <= 0 || !pos) return SQC_DUMP_ERR_ARG;
+  if (reader->bg.joinable()) return SQC_DUMP_ERR_ARG;  // one batch in flight at a time
+  std::array<double, 9> c{};
+  const bool has_cell = cell != nullptr;
+  if (has_cell) std::copy(cell, cell + 9, c.begin());
+  reader->bg_rc = 0;
+  reader->bg_n = 0;
+  reader->bg = std::thread([reader, maxframes, natoms, pos, c, has_cell] {
+    reader->bg_rc = sqc_dump_read_frames(reader, maxframes, natoms, has_cell ? c.data() : nullptr,
+                                         pos, &reader->bg_n, nullptr);
+  });
+  return SQC_DUMP_OK;
+}
+
+int sqc_dump_read_frames_end(sqc_dump_reader *reader, int64_t *nread) {
+  if (!reader || !nread) return SQC_DUMP_ERR_ARG;
+  if (!reader->bg.joinable()) return SQC_DUMP_ERR_ARG;
+  reader->bg.join();
+  *nread = reader->bg_n;
+  return reader->bg_rc;
+}
+
 int sqc_dump_close(sqc_dump_reader *reader) {
   if (!reader) return SQC_DUMP_OK;
+  if (reader->bg.joinable()) reader->bg.join();
   if (reader->base) munmap((void *)reader->base, reader->size);
   if (reader->fd >= 0) ::close(reader->fd);
   delete reader->pool;

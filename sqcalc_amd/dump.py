@@ -122,6 +122,31 @@ class DumpReader:
             self._raise(rc)
         return n.value, (_frame(h) if n.value else None)
 
+    def read_begin(self, out, natoms, cell=None):
+        """Start read_frames into out on a library thread and return at once;
+        out must stay alive and untouched until read_end."""
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.ndim != 3 \
+                or tuple(out.shape[1:]) != (natoms, 3):
+            raise ValueError("out must be a C-contiguous float64 (F, natoms, 3) array")
+        cbuf = None
+        if cell is not None:
+            cbuf = np.asfortranarray(np.asarray(cell, dtype=np.float64).reshape(3, 3))
+        self._pending = out
+        rc = self.lib.sqc_dump_read_frames_begin(self.handle, out.shape[0], natoms,
+                                                 cbuf.ctypes.data if cbuf is not None else None,
+                                                 out.ctypes.data)
+        if rc:
+            self._raise(rc)
+
+    def read_end(self):
+        """Wait for read_begin's batch; returns the number of frames read."""
+        n = ctypes.c_int64(0)
+        rc = self.lib.sqc_dump_read_frames_end(self.handle, ctypes.byref(n))
+        self._pending = None
+        if rc:
+            self._raise(rc)
+        return n.value
+
     def close(self):
         if self.handle:
             self.lib.sqc_dump_close(self.handle)

@@ -52,6 +52,8 @@ module sqc_hip_dump
       procedure :: open => reader_open
       procedure :: next_frame => reader_next
       procedure :: read_frames => reader_read_frames
+      procedure :: read_begin => reader_read_begin
+      procedure :: read_end => reader_read_end
       procedure :: message => reader_message
       procedure :: close => reader_close
    end type dump_reader_t
@@ -85,6 +87,21 @@ module sqc_hip_dump
          integer(c_int64_t), intent(out) :: nread
          type(c_dump_frame), intent(out) :: last
       end function sqc_dump_read_frames
+
+      integer(c_int) function sqc_dump_read_frames_begin(reader, maxframes, natoms, cell, pos) &
+         bind(C, name='sqc_dump_read_frames_begin')
+         import :: c_int, c_ptr, c_int64_t, c_double
+         type(c_ptr), value :: reader, pos
+         integer(c_int64_t), value :: maxframes, natoms
+         real(c_double), intent(in) :: cell(9)
+      end function sqc_dump_read_frames_begin
+
+      integer(c_int) function sqc_dump_read_frames_end(reader, nread) &
+         bind(C, name='sqc_dump_read_frames_end')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: reader
+         integer(c_int64_t), intent(out) :: nread
+      end function sqc_dump_read_frames_end
 
       integer(c_int) function sqc_dump_close(reader) bind(C, name='sqc_dump_close')
          import :: c_int, c_ptr
@@ -169,6 +186,29 @@ contains
       nread = got
       if (h%lineno > 0) self%lineno = h%lineno
    end subroutine reader_read_frames
+
+   !> Start reading up to maxframes frames into the memory at pos (3 natoms
+   !! doubles a frame) on a thread of the library; pos must stay allocated and
+   !! untouched until read_end.  The cell is copied before this returns.
+   subroutine reader_read_begin(self, pos, maxframes, natoms, cell, ierr)
+      class(dump_reader_t), intent(inout) :: self
+      type(c_ptr), intent(in) :: pos
+      integer(ik), intent(in) :: maxframes, natoms
+      real(rk), intent(in) :: cell(3, 3)
+      integer, intent(out) :: ierr
+      ierr = int(sqc_dump_read_frames_begin(self%handle, int(maxframes, c_int64_t), &
+                                            int(natoms, c_int64_t), reshape(cell, [9]), pos))
+   end subroutine reader_read_begin
+
+   !> Wait for the batch of read_begin: its frame count and status.
+   subroutine reader_read_end(self, nread, ierr)
+      class(dump_reader_t), intent(inout) :: self
+      integer(ik), intent(out) :: nread
+      integer, intent(out) :: ierr
+      integer(c_int64_t) :: got
+      ierr = int(sqc_dump_read_frames_end(self%handle, got))
+      nread = got
+   end subroutine reader_read_end
 
    !> Detail of the error that ended the reader, with its line.
    function reader_message(self) result(text)

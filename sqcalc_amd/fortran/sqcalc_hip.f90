@@ -47,10 +47,11 @@ program sqcalc_hip
    integer(c_int) :: rc
    integer :: ierr, nspec, a, b, unit_out, s
    integer(ik) :: natoms, j, nb, nframes, got
+   integer :: cur
    logical :: more
    integer(ik), allocatable :: spec_type(:), counts(:)
    integer(c_int32_t), allocatable, target :: species_of(:)
-   real(c_double), allocatable, target :: amp(:), xcoef(:, :), pos(:, :, :)
+   real(c_double), allocatable, target :: amp(:), xcoef(:, :), pos(:, :, :, :)
    real(c_double), allocatable, target :: num(:), part(:, :, :), den(:), gnum(:)
    integer(c_int64_t), allocatable :: shell_count(:)
    integer(c_int64_t) :: nfr
@@ -66,7 +67,9 @@ program sqcalc_hip
    end if
 
    ! --- first frame -------------------------------------------------------
-   call reader%open(input, ierr)
+   ! one parse thread fewer than the host's: the thread that waits on the
+   ! device spins, and a parse worker sharing its core slows every frame
+   call reader%open(input, ierr, -1)
    if (ierr /= 0) then
       write (error_unit, '(a)') 'sqcalc_hip: cannot open dump file "'//input//'"'
       stop 4
@@ -116,18 +119,29 @@ program sqcalc_hip
    rc = sqc_hip_plan_info(plan, info)
    if (.not. quiet) call report()
 
-   ! --- frames: the first one, then whole batches straight from the reader ---
-   allocate (pos(3, natoms, batch))
-   pos(:, :, 1) = frame%pos
-   nb = 1
+   ! --- frames: the first one, then whole batches straight from the reader.
+   ! Two buffers: the library parses batch k + 1 into one on a thread of its
+   ! own while batch k runs through the device from the other.
+   allocate (pos(3, natoms, batch, 2))
+   cur = 1
+   pos(:, :, 1, cur) = frame%pos
+   call reader%read_frames(pos(:, :, 2:batch, cur), natoms, ref_cell, got, ierr)
+   if (ierr /= 0) call reader_failure(ierr)
+   nb = 1 + got
    nframes = 0
    do
-      call reader%read_frames(pos(:, :, nb + 1:batch), natoms, ref_cell, got, ierr)
-      if (ierr /= 0) call reader_failure(ierr)
-      nb = nb + got
       more = nb == batch
+      if (more) then
+         call reader%read_begin(c_loc(pos(1, 1, 1, 3 - cur)), int(batch, ik), natoms, ref_cell, ierr)
+         if (ierr /= 0) call reader_failure(ierr)
+      end if
       call flush_batch()
       if (.not. more) exit
+      call reader%read_end(got, ierr)
+      if (ierr /= 0) call reader_failure(ierr)
+      cur = 3 - cur
+      nb = got
+      if (nb == 0) exit
    end do
    call reader%close()
 
@@ -189,7 +203,7 @@ contains
 
    subroutine flush_batch()
       if (nb == 0) return
-      rc = sqc_hip_accumulate(plan, pos, int(nb, c_int64_t))
+      rc = sqc_hip_accumulate(plan, pos(:, :, :, cur), int(nb, c_int64_t))
       if (rc /= sqc_hip_ok) then
          write (error_unit, '(a)') 'sqcalc_hip: '//sqc_hip_message(rc)//': '//sqc_hip_detail()
          stop 10

@@ -245,3 +245,32 @@ def test_read_frames_rejects_unsafe_buffers(tmp_path):
                 rd.read_frames(bad, 50)
         n, _ = rd.read_frames(np.zeros((2, 50, 3)), 50)
         assert n == 2
+
+
+def test_background_batch_read_matches_synchronous(tmp_path):
+    rng = np.random.default_rng(13)
+    a, traj = _trajectory(rng, 3000, 7, 18.0, 0.6)
+    p = _write(tmp_path, "bg", _dump_text(a, traj, "id type x y z"))
+    frames, _ = _read_all(p)
+    with DumpReader(p, 3) as rd:
+        got = []
+        bufs = [np.zeros((3, 3000, 3)), np.zeros((3, 3000, 3))]
+        k = 0
+        rd.read_begin(bufs[0], 3000, frames[0].cell)
+        while True:
+            n = rd.read_end()
+            got.extend(bufs[k % 2][:n].copy())
+            if n < 3:
+                break
+            k += 1
+            rd.read_begin(bufs[k % 2], 3000, frames[0].cell)
+    assert len(got) == 7
+    for g, f in zip(got, frames):
+        assert np.array_equal(g, f.pos)
+    # one batch in flight: a second begin before end is refused
+    with DumpReader(p, 2) as rd:
+        buf = np.zeros((2, 3000, 3))
+        rd.read_begin(buf, 3000)
+        with pytest.raises(DumpError):
+            rd.read_begin(np.zeros((2, 3000, 3)), 3000)
+        assert rd.read_end() == 2
