@@ -8,13 +8,12 @@ ierr code otherwise.  Positions come back continuous, as the reference's
 (F, N, 3) batch in the layout ``HipStructureFactor.accumulate`` takes, with the
 driver's per-frame checks of atom count and box (app/main.f90:240-247).
 
-``feed`` streams a whole file into a structure factor: one thread parses
-batch k + 1 into one half of a double buffer while the library runs batch k
-from the other (ctypes releases the GIL for both calls).
+``feed`` streams a whole file into a structure factor: the library parses
+batch k + 1 into one half of a double buffer on a thread of its own while
+batch k runs on the device from the other.
 """
 
 import ctypes
-import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -178,7 +177,9 @@ def _host_buffer(shape):
 
 
 def feed(sf, path, cell=None, batch=8, nthreads=0, max_frames=None):
-    """Stream the frames of a dump into ``sf.accumulate``, batch by batch.
+    """Stream the frames of a dump into ``sf.accumulate``, batch by batch:
+    the library parses batch k + 1 on a thread of its own (read_begin) while
+    batch k runs on the device.
 
     Every frame must keep ``sf.natoms`` atoms and the cell (3, 3) of ``cell``,
     or of the file's first frame when ``cell`` is None.  nthreads <= 0 parses
@@ -193,41 +194,29 @@ def feed(sf, path, cell=None, batch=8, nthreads=0, max_frames=None):
     with DumpReader(path, nthreads) as rd:
         natoms = sf.natoms
         bufs = [_host_buffer((batch, natoms, 3)), _host_buffer((batch, natoms, 3))]
-        result = {}
-
-        def parse(k, cell):
-            try:
-                buf = bufs[k % 2]
-                if cell is None:
-                    n1, h1 = rd.read_frames(buf[:1], natoms)
-                    if n1 == 0:
-                        result[k] = (0, None)
-                        return
-                    cell = h1.cell
-                    n2, _ = rd.read_frames(buf[1:], natoms, cell)
-                    result[k] = (1 + n2, cell)
-                else:
-                    result[k] = (rd.read_frames(buf, natoms, cell)[0], cell)
-            except Exception as exc:  # re-raised on the caller's thread
-                result[k] = exc
-
-        parse(0, cell)
+        # the first batch in the foreground; its first frame fixes the cell
+        if cell is None:
+            n1, h1 = rd.read_frames(bufs[0][:1], natoms)
+            if n1 == 0:
+                return 0
+            cell = h1.cell
+            n = 1 + rd.read_frames(bufs[0][1:], natoms, cell)[0]
+        else:
+            n = rd.read_frames(bufs[0], natoms, cell)[0]
         done, k = 0, 0
         while True:
-            got = result.pop(k)
-            if isinstance(got, Exception):
-                raise got
-            n, cell = got
             if max_frames is not None:
                 n = min(n, max_frames - done)
-            th = None
-            if n == batch and (max_frames is None or done + n < max_frames):
-                th = threading.Thread(target=parse, args=(k + 1, cell))
-                th.start()
-            if n:
-                sf.accumulate(bufs[k % 2][:n])
-                done += n
-            if th is None:
+            more = n == batch and (max_frames is None or done + n < max_frames)
+            if more:
+                rd.read_begin(bufs[(k + 1) % 2], natoms, cell)
+            try:
+                if n:
+                    sf.accumulate(bufs[k % 2][:n])
+                    done += n
+            finally:
+                nxt = rd.read_end() if more else 0
+            if not more or nxt == 0:
                 return done
-            th.join()
+            n = nxt
             k += 1
