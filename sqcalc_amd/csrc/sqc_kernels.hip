@@ -1170,8 +1170,13 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
       }
       double tr, ti;
       if (F == 1) {
-        tr = rr[0];
-        ti = ri[0];
+        // one field: neutron and unit weights ride in the atom strengths; a
+        // single species under X-ray weights carries strength 1 and takes
+        // f(|q|) here
+        const double w0 =
+            (FC != 1 && g.weight_kind == 2) ? xray_f(g.xcoef[0], qlen_of(g, h1, h2, h3)) : 1.0;
+        tr = w0 * rr[0];
+        ti = w0 * ri[0];
       } else {
         const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
         tr = 0.0;
@@ -1186,7 +1191,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
       const double val = tr * tr + ti * ti;
       atomicAdd(&acc[s], 2.0 * val);
       if (gnum) gnum[c] += val;
-      if (g.partials && F > 1) {
+      if (g.partials) {
+        // pairs a <= b of the per-species fields (one species: its own pair)
         int pidx = 1;
 #pragma unroll
         for (int a = 0; a < FM && a < F; ++a)
@@ -1196,7 +1202,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
       }
       // per-mode pair sums (one thread owns candidate c within a frame), kept
       // out of the loop above so the plain path compiles as before
-      if (gpart && F > 1) {
+      if (gpart) {
         int q = 0;
 #pragma unroll
         for (int a = 0; a < FM && a < F; ++a)
@@ -1354,8 +1360,10 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       }
       double tr = 0.0, ti = 0.0;
       if (F == 1) {
-        tr = rr[0];
-        ti = ri[0];
+        // as in the forward epilogue: f(|q|) of a single X-ray species
+        w[0] = (FC != 1 && g.weight_kind == 2) ? xray_f(g.xcoef[0], qlen_of(g, h1, h2, h3)) : 1.0;
+        tr = w[0] * rr[0];
+        ti = w[0] * ri[0];
       } else {
         const double q = g.weight_kind == 2 ? qlen_of(g, h1, h2, h3) : 0.0;
 #pragma unroll
@@ -1368,7 +1376,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       const double G = Gsh[s];
 #pragma unroll
       for (int cf = 0; cf < FM && cf < Fa; ++cf) {
-        const double wc = (Fa == 1) ? 1.0 : w[cf];
+        const double wc = w[cf];
         double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
         if (gP) {
           // pair gradients exist only with one field per species
@@ -1776,9 +1784,13 @@ template <typename Fn> static void pick_chunk(int tx2, Fn &&fn) {
 }
 
 // Field counts the x-pass epilogue is compiled for; others run its loops
-// with the count read at run time.
-template <typename Fn> static void pick_fields(int F, Fn &&fn) {
-  if (F == 1) return fn(std::integral_constant<int, 1>{});
+// with the count read at run time.  The one-field instance serves neutron
+// and unit weights only (their weights ride in the atom strengths); a
+// single X-ray species, whose f(|q|) the epilogue applies, takes the
+// run-time instance.
+template <typename Fn> static void pick_fields(const Geo &g, Fn &&fn) {
+  const int F = g.F;
+  if (F == 1 && g.weight_kind != 2) return fn(std::integral_constant<int, 1>{});
   if (F == 2) return fn(std::integral_constant<int, 2>{});
   fn(std::integral_constant<int, 0>{});
 }
@@ -1858,7 +1870,7 @@ static int forward_x_t(Plan &p, const void *W2, hipStream_t st, int mode, double
   {
     StageTimer t(p, 3);
     FixedSizes<T>::pick(g.n[0], [&](auto n) {
-      pick_fields(g.F, [&](auto fc) {
+      pick_fields(g, [&](auto fc) {
         auto go = [&](auto lbc) {
           hipLaunchKernelGGL(
               (k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, decltype(lbc)::value>),
@@ -1924,7 +1936,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   {
     StageTimer t(p, 5);
     FixedSizes<T>::pick(g.n[0], [&](auto n) {
-      pick_fields(g.F, [&](auto fc) {
+      pick_fields(g, [&](auto fc) {
         hipLaunchKernelGGL((k_adj_x<T, decltype(n)::value, decltype(fc)::value>), dim3(p.nblk3),
                            dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups, Fa, (const V *)rho,
                            (int64_t)p.ncand, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
@@ -2003,11 +2015,11 @@ static int lds_limits_t(Plan &p) {
     });
   });
   FixedSizes<T>::pick(p.geo.n[0], [&](auto n) {
-    pick_fields(p.geo.F, [&](auto fc) {
+    pick_fields(p.geo, [&](auto fc) {
       set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, 0>, p.lds3);
       set((const void *)k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, 2>, p.lds3);
     });
-    pick_fields(p.geo.F, [&](auto fc) {
+    pick_fields(p.geo, [&](auto fc) {
       set((const void *)k_adj_x<T, decltype(n)::value, decltype(fc)::value>, p.lds3);
     });
   });
@@ -2022,7 +2034,7 @@ static void persistent_grids_t(Plan &p) {
   const Geo &g = p.geo;
   int nx = 0;
   FixedSizes<T>::pick(g.n[0], [&](auto n) {
-    pick_fields(g.F, [&](auto fc) {
+    pick_fields(g, [&](auto fc) {
       auto query = [&](auto lbc) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &nx, k_fft_x_bin<T, decltype(n)::value, decltype(fc)::value, decltype(lbc)::value>,

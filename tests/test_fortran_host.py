@@ -229,3 +229,57 @@ def test_xrd_table_matches_python_host(exe, hip_device, tmp_path, partials, step
     np.testing.assert_allclose(g[:, 0], w[:, 0], atol=1e-9)
     assert np.abs(g[:, 1:] - w[:, 1:]).max() <= 1e-9 * np.abs(w[:, 1:]).max()
     sf.close()
+
+
+@pytest.mark.gpu
+def test_xrd_fcc_nickel_lines(exe, hip_device, tmp_path):
+    """Closed-form powder pattern: a perfect fcc Ni crystal (a = 3.52 A, 20^3
+    cells, 32,000 atoms) at Cu K-alpha over 40-80 deg in 0.2 deg bins, the
+    case the reference's own suite runs (test/test_sqcalc.py:936-1000).  Only
+    the (111), (200) and (220) reflections fall in the range; their bin
+    intensities relative to (111) must follow multiplicity x |f_Ni(q)|^2 x LP
+    (the structure factor of fcc is 4 f for all-even or all-odd hkl), every
+    other bin must stay empty, and with one species the pair column is the
+    total."""
+    from sqcalc_amd import elements, xrd
+    a, side, lam = 3.52, 20, 1.541838
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+    n = len(pos)
+    p = tmp_path / "ni.dump"
+    with open(p, "w") as fh:
+        fh.write("ITEM: TIMESTEP\n0\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n" % n)
+        fh.write("0 %.10f\n" % (a * side) * 3)
+        fh.write("ITEM: ATOMS id type x y z\n")
+        fh.write("".join("%d 1 %.10f %.10f %.10f\n" % (i + 1, *pos[i]) for i in range(n)))
+    out = tmp_path / "ni.xrd"
+    print("case fortran xrd fcc Ni", flush=True)
+    r = run(["static", "-i", str(p), "-m", "1:Ni", "-w", "xray", "--xrd", str(out),
+             "--xrd-lambda", str(lam), "--xrd-range", "40", "80", "--xrd-step", "0.2",
+             "--quiet", str(tmp_path / "sq.txt")])
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().splitlines()
+    assert lines[0].startswith("# xrd: lambda 1.541838 A") and "200 bins" in lines[0]
+    assert "Lorentz-polarization factor" in lines[1]
+    assert lines[3].split() == ["#", "2theta[deg]", "I", "I(Ni-Ni)"]
+    tab = np.array([[float(v) for v in l.split()] for l in lines[4:]])
+    assert tab.shape == (200, 3)
+    assert np.abs(tab[:, 1] - tab[:, 2]).max() <= 1e-9 * np.abs(tab[:, 1]).max()
+    c = elements.XRAY["Ni"]
+
+    def f_ni(q):
+        st = (q / (4.0 * np.pi)) ** 2
+        return c[8] + sum(c[k] * np.exp(-c[4 + k] * st) for k in range(4))
+
+    want, got = [], []
+    for hkl, mult in (((1, 1, 1), 8), ((2, 0, 0), 6), ((2, 2, 0), 12)):
+        q = np.sqrt(sum(h * h for h in hkl)) * 2.0 * np.pi / a
+        tth = float(xrd.two_theta_deg(q, lam))
+        j = int(np.argmin(np.abs(tab[:, 0] - tth)))
+        assert abs(tab[j, 0] - tth) <= 0.1 + 1e-9, (hkl, tth, tab[j, 0])
+        want.append(mult * f_ni(q) ** 2 * float(xrd.lorentz_polarization(tth)))
+        got.append(tab[j, 1])
+    for w, g in zip(want[1:], got[1:]):
+        assert abs(g / got[0] - w / want[0]) <= 0.01 * (w / want[0]), (g / got[0], w / want[0])
+    assert np.sort(tab[:, 1])[-4] <= 1e-6 * tab[:, 1].max()
