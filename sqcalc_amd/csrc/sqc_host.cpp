@@ -451,6 +451,11 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   const bool per_species = cfg->partials || cfg->weight_kind == SQC_HIP_WEIGHT_XRAY;
   g.F = per_species ? S : 1;
   g.ncomp = 1 + ((cfg->partials && S > 0) ? S * (S + 1) / 2 : 0);
+  // a single species under neutron or unit weights is spread with its weight
+  // as the strength; its pair column is of the unweighted amplitude
+  g.pair_scale = 1.0;
+  if (g.F == 1 && cfg->weight_kind != SQC_HIP_WEIGHT_XRAY)
+    g.pair_scale = g.amp[0] != 0.0 ? 1.0 / (g.amp[0] * g.amp[0]) : 0.0;
 
   // --- cell and modes ---
   const double two_pi = 2.0 * std::acos(-1.0);

@@ -72,6 +72,10 @@ struct Geo {
   int partials;
   int64_t natoms;
   double amp[kMaxSpecies];
+  // factor on the pair sums: 1, or 1 / amp[0]^2 for a one-field plan under
+  // neutron or unit weights, whose single field carries the species weight
+  // in its strength (the pair columns are of the unweighted amplitudes)
+  double pair_scale;
   double xcoef[kMaxSpecies][9];
   int nxcd;         // XCDs the workgroups are dealt to
 };

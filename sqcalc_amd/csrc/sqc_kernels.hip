@@ -1155,6 +1155,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
     for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
     __syncthreads();
     // candidate c of line li, kept in shell s
+    // the factor on the pair sums (1 whenever there are two or more fields)
+    const double ps = FC == 2 ? 1.0 : g.pair_scale;
     auto bin_mode = [&](int c, int li, unsigned s) {
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
       const int h1 = lg.h[li][2] + (c - lg.off[li]);
@@ -1191,24 +1193,24 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
       const double val = tr * tr + ti * ti;
       atomicAdd(&acc[s], 2.0 * val);
       if (gnum) gnum[c] += val;
-      if (g.partials) {
+      if (FC != 1 && g.partials) {
         // pairs a <= b of the per-species fields (one species: its own pair)
         int pidx = 1;
 #pragma unroll
         for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
           for (int b = a; b < FM && b < F; ++b, ++pidx)
-            atomicAdd(&acc[pidx * nq + s], 2.0 * (rr[a] * rr[b] + ri[a] * ri[b]));
+            atomicAdd(&acc[pidx * nq + s], 2.0 * ps * (rr[a] * rr[b] + ri[a] * ri[b]));
       }
       // per-mode pair sums (one thread owns candidate c within a frame), kept
       // out of the loop above so the plain path compiles as before
-      if (gpart) {
+      if (FC != 1 && gpart) {
         int q = 0;
 #pragma unroll
         for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
           for (int b = a; b < FM && b < F; ++b, ++q)
-            gpart[(size_t)q * ncand + c] += rr[a] * rr[b] + ri[a] * ri[b];
+            gpart[(size_t)q * ncand + c] += ps * (rr[a] * rr[b] + ri[a] * ri[b]);
       }
     };
     // one copy of the body: the prefetched index of round k is picked by a
@@ -1344,6 +1346,8 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       }
     }
     // candidate c of line li, kept in shell s
+    // the factor on the pair sums (1 whenever there are two or more fields)
+    const double ps = FC == 2 ? 1.0 : g.pair_scale;
     auto amplify = [&](int c, int li, unsigned s, V r0) {
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
       const int h1 = lg.h[li][2] + (c - lg.off[li]);
@@ -1378,14 +1382,14 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       for (int cf = 0; cf < FM && cf < Fa; ++cf) {
         const double wc = w[cf];
         double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
-        if (gP) {
+        if (FC != 1 && gP) {
           // pair gradients exist only with one field per species
           const double ic = inc[s];
 #pragma unroll
           for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
             for (int b = a; b < FM && b < F; ++b) {
-              const double H = gP[((size_t)a * S + b) * nq + s] * ic;
+              const double H = gP[((size_t)a * S + b) * nq + s] * ic * ps;
               if (cf == a) { cr += H * rr[b]; ci -= H * ri[b]; }
               if (cf == b) { cr += H * rr[a]; ci -= H * ri[a]; }
             }
@@ -1784,13 +1788,13 @@ template <typename Fn> static void pick_chunk(int tx2, Fn &&fn) {
 }
 
 // Field counts the x-pass epilogue is compiled for; others run its loops
-// with the count read at run time.  The one-field instance serves neutron
-// and unit weights only (their weights ride in the atom strengths); a
-// single X-ray species, whose f(|q|) the epilogue applies, takes the
-// run-time instance.
+// with the count read at run time.  The one-field instance serves one
+// weighted field with no pair sums (neutron or unit weights, partials off):
+// a single X-ray species, whose f(|q|) the epilogue applies, and a single
+// species with its pair column take the run-time instance.
 template <typename Fn> static void pick_fields(const Geo &g, Fn &&fn) {
   const int F = g.F;
-  if (F == 1 && g.weight_kind != 2) return fn(std::integral_constant<int, 1>{});
+  if (F == 1 && g.weight_kind != 2 && !g.partials) return fn(std::integral_constant<int, 1>{});
   if (F == 2) return fn(std::integral_constant<int, 2>{});
   fn(std::integral_constant<int, 0>{});
 }

@@ -159,10 +159,12 @@ def test_loss_step_under_graph_capture(hip_device, precision):
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-def test_single_species_xray(hip_device, precision):
-    """One species under X-ray weights has one field of strength 1, so f(|q|)
-    enters in the x-pass epilogue and in the adjoint's amplification; the
-    S(q) table, the per-mode sums and dL/dr must match the oracle."""
+@pytest.mark.parametrize("weight", ["xray", "neutron"])
+def test_single_species(hip_device, precision, weight):
+    """One species has one field.  Under X-ray weights its strength is 1 and
+    f(|q|) enters in the x-pass epilogue and the adjoint's amplification;
+    under neutron weights b rides in the strength.  The S(q) table with its
+    single pair column and dL/dr must match the oracle."""
     import torch
 
     from sqcalc_amd import HipStructureFactor
@@ -172,17 +174,19 @@ def test_single_species_xray(hip_device, precision):
                                               ntypes=1, tilt=(0.6, 0.2, -0.3))
     mapping = {1: "Ni"}
     qmax, nq = 5.0, 40
-    print("case single-species xray", precision, flush=True)
+    print("case single species", weight, precision, flush=True)
     eps = 1e-11 if precision == "double" else 1e-6
-    sf = HipStructureFactor(cell, origin, types, mapping, weight="xray", norm="mean", qmax=qmax,
+    sf = HipStructureFactor(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax,
                             nq=nq, precision=precision, eps=eps, want_grid=True)
     sf.accumulate(frames)
-    run = O.Run(cell, origin, types, mapping, weight="xray", norm="mean", qmax=qmax, nq=nq)
+    run = O.Run(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax, nq=nq)
     for f in frames:
         run.add_frame(f)
     ref = run.table()
     got = sf.table()
     scale = np.maximum(np.abs(ref[:, 1:]), 1.0)
+    assert got.shape[1] == 3          # q, S, S(Ni-Ni)
+    assert np.abs(got[:, 2]).max() > 0
     assert np.abs(got[:, 1:] - ref[:, 1:]).max() / scale.max() < (1e-9 if precision == "double"
                                                                    else 1e-6)
     rng = np.random.default_rng(2)
@@ -190,10 +194,10 @@ def test_single_species_xray(hip_device, precision):
     pos = torch.tensor(frames[0], device=hip_device, requires_grad=True)
     sq_matching_loss(pos, sf, target).backward()
     rho = O.direct_rho(frames[0], cell, origin, run.species_of, 1, run.md)
-    num, _ = O.shell_sums(rho, run.md, "xray", run.amp, run.xcoef, partials=False)
+    num, _ = O.shell_sums(rho, run.md, weight, run.amp, run.xcoef, partials=False)
     S = np.where(run.den > 0, num / np.where(run.den > 0, run.den, 1.0), 0.0)
     gS = 2.0 * (S - target.cpu().numpy())
-    g_ref = O.gradient(frames[0], cell, origin, run.species_of, run.md, rho, "xray", run.amp,
+    g_ref = O.gradient(frames[0], cell, origin, run.species_of, run.md, rho, weight, run.amp,
                        run.xcoef, gS, run.den, run.shell_count)
     g = pos.grad.cpu().numpy()
     err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
