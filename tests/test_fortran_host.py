@@ -283,3 +283,33 @@ def test_xrd_fcc_nickel_lines(exe, hip_device, tmp_path):
     for w, g in zip(want[1:], got[1:]):
         assert abs(g / got[0] - w / want[0]) <= 0.01 * (w / want[0]), (g / got[0], w / want[0])
     assert np.sort(tab[:, 1])[-4] <= 1e-6 * tab[:, 1].max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("weight", ["neutron", "xray"])
+def test_single_species_table_matches_python_host(exe, hip_device, tmp_path, weight):
+    """One atom type: the S(q) table and its S(Ni-Ni) column through the
+    command against the Python host on the frames the reader returned."""
+    from sqcalc_amd import HipStructureFactor
+    from sqcalc_amd.dump import DumpReader
+    from helpers import random_case
+    cell, origin, types, frames = random_case(natoms=400, length=15.0, nframes=2, seed=3,
+                                              ntypes=1)
+    p = tmp_path / "one.dump"
+    write_dump(p, frames, types, [0.0] * 3, [15.0] * 3)
+    out = tmp_path / "sq.txt"
+    print("case fortran single species", weight, flush=True)
+    r = run(["static", "-i", str(p), "-m", "1:Ni", "-w", weight, "--qmax", "5", "--nq", "40",
+             "--quiet", str(out)])
+    assert r.returncode == 0, r.stderr
+    header, rows = read_table(out.read_text())
+    assert header.split()[:4] == ["#", "q", "S(q)", "S(Ni-Ni)"]
+    with DumpReader(p, 2) as rd:
+        read = list(rd)
+    sf = HipStructureFactor(read[0].cell, read[0].origin, read[0].type_id, {1: "Ni"},
+                            weight=weight, qmax=5.0, nq=40)
+    sf.accumulate(np.stack([fr.pos for fr in read]))
+    ref = sf.table()
+    assert np.abs(ref[:, 2]).max() > 0
+    assert rel_dev(ref, rows) < 1e-9
+    sf.close()
