@@ -365,7 +365,10 @@ int alloc_bins(sqc::Plan &p) {
 // accumulators and the 1 / phihat_1 table.
 size_t xpass_lds(const sqc::Plan &p, int lb) {
   const sqc::Geo &g = p.geo;
-  const size_t accb = ((size_t)g.ncomp * g.nq + g.M[0] + 1) * sizeof(double);
+  // shell sums (ncomp rows of nq) unless they live in HBM; the adjoint x
+  // pass keeps one row of nq there, two with pair gradients (ncomp >= 2)
+  const size_t rows = g.acc_global ? 2 : (size_t)g.ncomp;
+  const size_t accb = (rows * g.nq + g.M[0] + 1) * sizeof(double);
   return ((size_t)lb * g.F * g.rs1 + g.n[0]) * vsize(p) + accb;
 }
 
@@ -641,6 +644,9 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
     while (tx2 > 2 && ypass_lds(p, tx2) > lds_pref_y(p)) tx2 /= 2;  // the y pass is compiled for 2..16
     set_tiles(p, kTiles[pick][0], kTiles[pick][1], tx2);
   }
+  // shell sums in HBM when even one x line per group leaves them no room
+  g.acc_global = 0;
+  if (xpass_lds(p, 1) > lds_cap) g.acc_global = 1;
   int lb0 = sqc::kMaxLb;
   while (lb0 > 1 && xpass_lds(p, lb0) > lds_pref) --lb0;
   if (xpass_lds(p, lb0) > lds_cap)

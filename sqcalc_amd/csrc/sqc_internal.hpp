@@ -76,6 +76,9 @@ struct Geo {
   // neutron or unit weights, whose single field carries the species weight
   // in its strength (the pair columns are of the unweighted amplitudes)
   double pair_scale;
+  // 1: the x pass keeps its shell sums in its own row of the per-workgroup
+  // partials in HBM (FP64 atomics) because ncomp * nq doubles exceed the LDS
+  int acc_global;
   double xcoef[kMaxSpecies][9];
   int nxcd;         // XCDs the workgroups are dealt to
 };

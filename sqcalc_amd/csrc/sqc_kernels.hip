@@ -1118,8 +1118,14 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
   const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
   V *tw = rows + (size_t)lb * F * rs;
-  double *acc = reinterpret_cast<double *>(tw + n1);
-  double *inv1 = acc + (size_t)ncomp * nq;  // 1 / phihat_1(|h1|)
+  // shell sums in LDS, or, when ncomp * nq doubles do not fit there, in this
+  // workgroup's own row of wgpart (zeroed here; the group loop's barrier
+  // orders that before the first add)
+  const bool gacc = g.acc_global != 0;
+  double *acc = gacc ? wgpart + (size_t)blockIdx.x * ncomp * nq
+                     : reinterpret_cast<double *>(tw + n1);
+  double *inv1 = gacc ? reinterpret_cast<double *>(tw + n1)
+                      : acc + (size_t)ncomp * nq;  // 1 / phihat_1(|h1|)
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
@@ -1228,6 +1234,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
     }
   }
   __syncthreads();
+  if (gacc) return;  // the sums are already in the row
   double *out = wgpart + (size_t)blockIdx.x * ncomp * nq;
   for (int i = tid; i < ncomp * nq; i += NT) out[i] = acc[i];
 }

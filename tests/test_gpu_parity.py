@@ -124,3 +124,26 @@ def test_dense_windows_span_many_staging_rounds(hip_device, precision):
         print("  partials=%s: max deviation %.2e" % (partials, dev))
         assert dev < (1e-9 if precision == "double" else 2e-5)
         sf.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_eight_species_shell_sums_in_hbm(hip_device, precision):
+    """8 species with partials at 500 shells: 37 rows of shell sums (148 KB
+    of FP64) leave no LDS for even one x line, so the x pass keeps them in
+    its rows of the per-workgroup partials in HBM.  Every column against the
+    oracle (eps 1e-10 in FP64 and 1e-6 in FP32: with 60 atoms a species the
+    pair columns are small, and this case checks the accumulation path, not
+    the FP32 default's accuracy, which the golden tables pin)."""
+    cell, origin, types, frames = random_case(natoms=480, length=12.0, nframes=2, seed=17,
+                                              ntypes=8, tilt=(0.5, 0.0, -0.3))
+    print("case eight species", precision, flush=True)
+    sf = _hip(cell, origin, types, None, "unit", "mean", 0.0, 4.0, 500, precision,
+              eps=1e-10 if precision == "double" else 1e-6)
+    sf.accumulate(frames)
+    table = sf.table()
+    assert table.shape[1] == 2 + 36
+    sf.close()
+    ref = oracle_run(cell, origin, types, None, "unit", "mean", 0.0, 4.0, 500, frames).table()
+    devs = [rel_dev(ref[:, c], table[:, c]) for c in range(1, ref.shape[1])]
+    print("  worst column %.2e" % max(devs))
+    assert max(devs) <= TOL[precision]
