@@ -1121,13 +1121,31 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
   // shell sums in LDS, or, when ncomp * nq doubles do not fit there, in this
   // workgroup's own row of wgpart (zeroed here; the group loop's barrier
   // orders that before the first add)
-  const bool gacc = g.acc_global != 0;
-  double *acc = gacc ? wgpart + (size_t)blockIdx.x * ncomp * nq
-                     : reinterpret_cast<double *>(tw + n1);
-  double *inv1 = gacc ? reinterpret_cast<double *>(tw + n1)
-                      : acc + (size_t)ncomp * nq;  // 1 / phihat_1(|h1|)
+  // (two pointers, not one selected at run time, so that the LDS adds stay
+  // LDS instructions)
+  // only the run-time field-count instance keeps sums in HBM (the host
+  // routes every such plan there), so the compiled instances keep pure LDS adds
+  const bool gacc = FC == 0 && g.acc_global != 0;
+  double *acc = reinterpret_cast<double *>(tw + n1);
+  double *accg = wgpart + (size_t)blockIdx.x * ncomp * nq;
+  double *inv1 = gacc ? acc : acc + (size_t)ncomp * nq;  // 1 / phihat_1(|h1|)
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
+  if (gacc)
+    for (int i = tid; i < ncomp * nq; i += NT) accg[i] = 0.0;
+  else
+    for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
+  // address spaces spelled out: the two adds must not merge into one generic
+  // (flat) atomic on a selected pointer
+  typedef __attribute__((address_space(3))) double lds_double;
+  typedef __attribute__((address_space(1))) double hbm_double;
+  auto add = [&](int i, double v) {
+    if (gacc)
+      __hip_atomic_fetch_add((hbm_double *)(accg + i), v, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    else
+      __hip_atomic_fetch_add((lds_double *)(acc + i), v, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
   for (int i = tid; i <= M1; i += NT) inv1[i] = 1.0 / ph1[i];
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
@@ -1197,7 +1215,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
         }
       }
       const double val = tr * tr + ti * ti;
-      atomicAdd(&acc[s], 2.0 * val);
+      add(s, 2.0 * val);
       if (gnum) gnum[c] += val;
       if (FC != 1 && g.partials) {
         // pairs a <= b of the per-species fields (one species: its own pair)
@@ -1206,7 +1224,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
         for (int a = 0; a < FM && a < F; ++a)
 #pragma unroll
           for (int b = a; b < FM && b < F; ++b, ++pidx)
-            atomicAdd(&acc[pidx * nq + s], 2.0 * ps * (rr[a] * rr[b] + ri[a] * ri[b]));
+            add(pidx * nq + s, 2.0 * ps * (rr[a] * rr[b] + ri[a] * ri[b]));
       }
       // per-mode pair sums (one thread owns candidate c within a frame), kept
       // out of the loop above so the plain path compiles as before
@@ -1801,6 +1819,7 @@ template <typename Fn> static void pick_chunk(int tx2, Fn &&fn) {
 // species with its pair column take the run-time instance.
 template <typename Fn> static void pick_fields(const Geo &g, Fn &&fn) {
   const int F = g.F;
+  if (g.acc_global) return fn(std::integral_constant<int, 0>{});  // shell sums in HBM
   if (F == 1 && g.weight_kind != 2 && !g.partials) return fn(std::integral_constant<int, 1>{});
   if (F == 2) return fn(std::integral_constant<int, 2>{});
   fn(std::integral_constant<int, 0>{});
