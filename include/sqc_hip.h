@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SQC_HIP_ABI_VERSION 1
+#define SQC_HIP_ABI_VERSION 2
 
 /* Return codes.  Values 1..20 keep the meaning of FINUFFT's codes where one
  * applies (external/finufft/include/finufft_errors.h:5-31). */
@@ -77,7 +77,9 @@ typedef struct sqc_hip_config {
                                kept mode (the pair sums behind the --xrd pair
                                columns, src/sqc_structure_factor.f90:538-588) */
   const int32_t *species_of;  /* [natoms], 0-based species of each atom */
-  const double *amp_const;    /* [nspecies] unit/neutron amplitudes */
+  const double *amp_const;    /* [nspecies] neutron amplitudes (ignored, and may be
+                                 NULL, under unit and X-ray weights: the X-ray
+                                 amplitude is f_a(|q|) from xray_coef alone) */
   const double *xray_coef;    /* [nspecies][9] IT92 a1..a4, b1..b4, c */
   int32_t tune;             /* 1 = time the spread tiles, y-pass chunks and x-pass
                                line groups on the first frame, keep the fastest */
@@ -104,6 +106,10 @@ typedef struct sqc_hip_info {
   int32_t tile[3];          /* spread tile (x, y) and y-pass column chunk */
   int32_t ncomp;            /* 1 + species pairs: rows of a per-frame output */
   int64_t rho_len;          /* complex values per field in a saved rho buffer */
+  int32_t shell_sums_in_hbm; /* 1: the x pass adds its shell sums in its own row of
+                               HBM (ncomp * nq doubles do not fit the LDS beside one
+                               x line); 0: in LDS */
+  int32_t reserved_info;
 } sqc_hip_info;
 
 /* Number of visible HIP devices (cudaGetDeviceCount in src/sqc_cufinufft.f90:143-174). */

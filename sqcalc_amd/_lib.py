@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # SQC_HIP_LIBRARY selects another build of the library (A/B measurements)
 LIB_PATH = os.environ.get("SQC_HIP_LIBRARY", os.path.join(HERE, "libsqc_hip.so"))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 F64, F32 = 0, 1
 WEIGHT = {"unit": 0, "neutron": 1, "xray": 2}
 STAGES = ["atom_sort", "spread_z", "dft_y", "dft_x_bin", "finalize", "adj_x", "adj_y",
@@ -42,7 +42,8 @@ class Info(_c.Structure):
         ("fine", _c.c_int64 * 3), ("nspread", _c.c_int32), ("nfields", _c.c_int32),
         ("beta", _c.c_double), ("upsampfac", _c.c_double), ("device_bytes", _c.c_int64),
         ("nlines", _c.c_int64), ("tile", _c.c_int32 * 3), ("ncomp", _c.c_int32),
-        ("rho_len", _c.c_int64),
+        ("rho_len", _c.c_int64), ("shell_sums_in_hbm", _c.c_int32),
+        ("reserved_info", _c.c_int32),
     ]
 
 

@@ -39,6 +39,18 @@ def _norms(sf, device):
     return cache[key]
 
 
+def _pair_rows(sf, device):
+    """Row a * S + b of every pair a <= b in the order of the P rows, as a
+    device index tensor built once per plan and device."""
+    cache = sf.__dict__.setdefault("_device_pair_rows", {})
+    key = str(device)
+    if key not in cache:
+        S = len(sf.species_type)
+        rows = [a * S + b for a in range(S) for b in range(a, S)]
+        cache[key] = torch.tensor(rows, dtype=torch.int64, device=device)
+    return cache[key]
+
+
 class _SqFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pos, sf):
@@ -57,6 +69,9 @@ class _SqFunction(torch.autograd.Function):
         S = out[0] * inv_den
         P = out[1:] * inv_pair
         ctx.sf = sf
+        # an output the loss does not use comes back as None rather than as
+        # zeros, so a loss on S alone skips the pair terms of the adjoint
+        ctx.set_materialize_grads(False)
         ctx.save_for_backward(pos, rho)
         return S, P
 
@@ -68,14 +83,14 @@ class _SqFunction(torch.autograd.Function):
         gS = (gS if gS is not None else torch.zeros(nq, dtype=torch.float64,
                                                     device=pos.device)).contiguous()
         gP_full = None
-        if gP is not None and sf.partials and bool(torch.any(gP != 0)):
+        if gP is not None and sf.partials:
+            # the pair rows go to their (a, b <= a) places on the device, with
+            # no look at their values: a zero row adds nothing, and a host
+            # test of the values would synchronise (and break graph capture)
             S = len(sf.species_type)
-            gP_full = torch.zeros((S, S, nq), dtype=torch.float64, device=pos.device)
-            k = 0
-            for a in range(S):
-                for b in range(a, S):
-                    gP_full[a, b] = gP[k]
-                    k += 1
+            gP_full = torch.zeros((S * S, nq), dtype=torch.float64, device=pos.device)
+            gP_full.index_copy_(0, _pair_rows(sf, pos.device), gP.contiguous())
+            gP_full = gP_full.view(S, S, nq)
         grad = torch.empty_like(pos)
         rc = sf.lib.sqc_hip_adjoint_device(sf.plan, pos.data_ptr(), rho.data_ptr(),
                                            gS.data_ptr(),

@@ -279,6 +279,17 @@ size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
   return (size_t)(txr / 2) * ty * g.cs * vsize(p) + sqc::kStageCap * 16;
 }
 
+// The gather adds a per-wave scratch of the kernel values of the atoms of
+// one round: (64 / lanes per atom) atoms x 6 ns values.
+size_t gather_lds(const sqc::Plan &p, int txr, int ty) {
+  const int lpa = std::max(8, txr * ty);
+  return spread_lds(p, txr, ty) +
+         (size_t)(sqc::kThreads / 64) * (64 / lpa) * 6 * p.geo.ns * (vsize(p) / 2);
+}
+
+// LDS a CU offers the dynamic part of an x-pass workgroup.
+size_t xpass_cap() { return kLdsCap - sqc::static_lds_xpass(); }
+
 // y-line stride: the smallest odd stride that holds a line (n2 + n2/32 + 1
 // points, the DFT's padded stage layout).  An odd stride spreads the
 // transposed copies (lane (line, y) at line * ls2 + y) over the banks and
@@ -305,7 +316,9 @@ int ilog2i(int v) {
 bool tile_fits(const sqc::Plan &p, int txr, int ty, size_t limit) {
   const sqc::Geo &g = p.geo;
   const int n1 = g.n[0], n2 = g.n[1], ns = g.ns;
-  if (spread_lds(p, txr, ty) > limit) return false;
+  if (spread_lds(p, txr, ty) + sqc::static_lds_spread() > limit) return false;
+  // the gather holds the same tile plus its kernel-value scratch
+  if (gather_lds(p, txr, ty) + sqc::static_lds_spread() > kLdsCap) return false;
   if (2 * txr > n1 || 2 * ty > n2) return false;
   const int ntx = (n1 + txr - 1) / txr, nty = (n2 + ty - 1) / ty;
   const int hx = (int)std::ceil((0.5 * ns + 1.0) / txr) + (n1 % txr ? 1 : 0);
@@ -320,12 +333,7 @@ void set_tiles(sqc::Plan &p, int txr, int ty, int tx2) {
   g.txr = txr;
   g.ty = ty;
   p.lds1 = spread_lds(p, txr, ty);
-  {
-    // the gather adds a per-wave scratch of the kernel values of the atoms
-    // of one round: (64 / lanes per atom) atoms x 6 ns values
-    const int lpa = std::max(8, txr * ty);
-    p.lds1g = p.lds1 + (size_t)(sqc::kThreads / 64) * (64 / lpa) * 6 * g.ns * (vsize(p) / 2);
-  }
+  p.lds1g = gather_lds(p, txr, ty);
   g.ltxr = ilog2i(g.txr);
   g.lty = ilog2i(g.ty);
   g.ntx = (n1 + g.txr - 1) / g.txr;
@@ -377,7 +385,8 @@ size_t xpass_lds(const sqc::Plan &p, int lb) {
 void set_xpass(sqc::Plan &p, int lb) {
   p.lb = lb;
   p.lds3 = xpass_lds(p, lb);
-  const int resident = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsCap / p.lds3));
+  const int resident = (int)std::max<size_t>(
+      1, std::min<size_t>(8, kLdsCap / (p.lds3 + sqc::static_lds_xpass())));
   const int ngroups = (p.geo.nlines + lb - 1) / lb;
   p.nblk3 = std::max(1, std::min(ngroups, resident * p.ncu));
 }
@@ -438,8 +447,9 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   g.weight_kind = cfg->weight_kind;
   g.partials = cfg->partials ? 1 : 0;
   for (int a = 0; a < S; ++a) {
-    g.amp[a] = cfg->weight_kind == SQC_HIP_WEIGHT_UNIT ? 1.0
-               : cfg->amp_const ? cfg->amp_const[a] : 1.0;
+    // strengths: 1 under unit and X-ray weights (the X-ray amplitude is
+    // f_a(|q|), applied in the x-pass epilogue), b_a under neutron weights
+    g.amp[a] = cfg->weight_kind == SQC_HIP_WEIGHT_NEUTRON ? cfg->amp_const[a] : 1.0;
     for (int i = 0; i < 9; ++i) g.xcoef[a][i] = cfg->xray_coef ? cfg->xray_coef[9 * a + i] : 0.0;
   }
   p.species_of.assign(cfg->species_of, cfg->species_of + cfg->natoms);
@@ -617,7 +627,6 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
 
   // --- launch shapes ---
   const size_t vs = p.prec == 1 ? 8 : 16;
-  const size_t lds_cap = kLdsCap;
   const size_t lds_pref = ::lds_pref(p);
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
   for (int i = 0; i < 3; ++i)
@@ -646,10 +655,10 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   }
   // shell sums in HBM when even one x line per group leaves them no room
   g.acc_global = 0;
-  if (xpass_lds(p, 1) > lds_cap) g.acc_global = 1;
+  if (xpass_lds(p, 1) > xpass_cap()) g.acc_global = 1;
   int lb0 = sqc::kMaxLb;
   while (lb0 > 1 && xpass_lds(p, lb0) > lds_pref) --lb0;
-  if (xpass_lds(p, lb0) > lds_cap)
+  if (xpass_lds(p, lb0) > xpass_cap())
     return fail(SQC_HIP_ERR_MAXNALLOC, "x line x species too large for the LDS DFT");
   int dev_count = 0;
   HIP_TRY(hipGetDeviceCount(&dev_count), "hipGetDeviceCount");
@@ -901,7 +910,7 @@ int tune(sqc::Plan &p, const double *pos_dev) {
   if (int rc = time_stages(p, pos_dev, st)) return rc;
   best = st[3];
   for (int lb = sqc::kMaxLb; lb >= 1; lb /= 2) {
-    if (xpass_lds(p, lb) > kLdsCap || lb == blb) continue;
+    if (xpass_lds(p, lb) > xpass_cap() || lb == blb) continue;
     if (int rc = apply_tiles(p, btxr, bty, btx2, lb)) return rc;
     if (int rc = time_stages(p, pos_dev, st)) return rc;
     if (st[3] < kGain * best) {
@@ -1070,6 +1079,7 @@ int sqc_hip_plan_info(const sqc_hip_plan *plan, sqc_hip_info *info) {
   info->tile[2] = p.tx2;
   info->ncomp = p.geo.ncomp;
   info->rho_len = p.ncand;
+  info->shell_sums_in_hbm = p.geo.acc_global;
   return SQC_HIP_OK;
 }
 

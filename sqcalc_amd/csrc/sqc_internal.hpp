@@ -172,6 +172,11 @@ int launch_forward_x(Plan &p, const void *W2, hipStream_t st, int mode, double *
                      void *rho_out);
 int launch_finalize(Plan &p, double *frame_out, int accumulate, hipStream_t st);
 int kernel_lds_limits(Plan &p);
+// Static LDS (bytes) of the spread and gather kernels (their staging block)
+// and of the x passes (their line-group table): a launch's dynamic LDS must
+// leave room for it within the 160 KB of a CU.
+size_t static_lds_spread();
+size_t static_lds_xpass();
 void persistent_grids(Plan &p);
 int collect_stage_times(Plan &p);
 

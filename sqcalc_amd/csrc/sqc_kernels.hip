@@ -2091,6 +2091,9 @@ void persistent_grids(Plan &p) {
     persistent_grids_t<double>(p);
 }
 
+size_t static_lds_spread() { return sizeof(Stage); }
+size_t static_lds_xpass() { return sizeof(LineGroup); }
+
 int kernel_lds_limits(Plan &p) {
   return p.prec == 1 ? lds_limits_t<float>(p) : lds_limits_t<double>(p);
 }

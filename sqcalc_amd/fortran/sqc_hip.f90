@@ -10,7 +10,7 @@ module sqc_hip
    implicit none
    private
 
-   integer(c_int32_t), parameter, public :: sqc_hip_abi_version = 1
+   integer(c_int32_t), parameter, public :: sqc_hip_abi_version = 2
 
    integer(c_int), parameter, public :: sqc_hip_ok = 0
    integer(c_int), parameter, public :: sqc_hip_warn_eps_too_small = 1
@@ -70,6 +70,8 @@ module sqc_hip
       integer(c_int32_t) :: tile(3)
       integer(c_int32_t) :: ncomp
       integer(c_int64_t) :: rho_len
+      integer(c_int32_t) :: shell_sums_in_hbm
+      integer(c_int32_t) :: reserved_info
    end type sqc_hip_info_t
 
    public :: sqc_hip_device_count, sqc_hip_plan_create, sqc_hip_plan_info, sqc_hip_shells, &

@@ -313,3 +313,41 @@ def test_single_species_table_matches_python_host(exe, hip_device, tmp_path, wei
     assert np.abs(ref[:, 2]).max() > 0
     assert rel_dev(ref, rows) < 1e-9
     sf.close()
+
+
+@pytest.mark.gpu
+def test_c1_scaled_dump_through_fortran(exe, hip_device, tmp_path):
+    """The CPU-runnable configuration as stated: cube L = 20, N = 1000 (333 Si,
+    667 O), 10 frames written as a LAMMPS dump with scaled coordinates (xs ys
+    zs), unit weights, norm mean, partials, q_max 10, NUFFT tolerance 1e-9,
+    through `sqcalc_hip static` in FP64: every column within 1e-9 of the
+    oracle's direct sum over the positions the dump holds (the fractional
+    coordinates as written, %.10f)."""
+    from helpers import oracle_run
+    from oracle import oracle as O
+    from sqcalc_amd.synthetic import config
+    wl = config("c1")
+    frames = wl.frames()
+    lo, hi = [0.0] * 3, [20.0] * 3
+    p = tmp_path / "c1.dump"
+    write_dump(p, frames, wl.types, lo, hi, scaled=True)
+    cell, origin = O.cell_from_bounds(lo, hi)
+    inv = np.linalg.inv(cell)
+    held = []
+    for f in frames:
+        frac = (f - origin) @ inv.T
+        frac = np.array([float("%.10f" % v) for v in frac.ravel()]).reshape(frac.shape)
+        held.append(frac @ cell.T + origin)
+    out = tmp_path / "sq.txt"
+    print("case c1 fortran scaled dump", flush=True)
+    r = run(["static", "-i", str(p), "-m", "1:Si,2:O", "-w", "unit", "--norm", "mean",
+             "--qmax", "10", "--nq", "500", "--eps", "1e-9", "--quiet", str(out)])
+    assert r.returncode == 0, r.stderr
+    header, rows = read_table(out.read_text())
+    assert header.split()[:3] == ["#", "q", "S(q)"] and rows.shape == (500, 5)
+    ref = oracle_run(cell, origin, wl.types, wl.mapping, "unit", "mean", 0.0, 10.0, 500,
+                     held).table()
+    np.testing.assert_allclose(rows[:, 0], np.round(ref[:, 0], 6), atol=1e-12)
+    dev = rel_dev(ref[:, 1:], rows[:, 1:])
+    print("  max deviation %.2e" % dev)
+    assert dev < 1e-9

@@ -17,20 +17,26 @@ pytestmark = pytest.mark.gpu
 TOL = {"double": 1e-8, "single": 1e-5}
 
 
+MAPPINGS = {2: {1: "Si", 2: "O"}, 4: {1: "Si", 2: "O", 3: "Na", 4: "Ca"}}
+
+
 @pytest.mark.parametrize("precision", ["double", "single"])
-@pytest.mark.parametrize("weight,partials", [("neutron", False), ("unit", True),
-                                             ("xray", True), ("xray", False)])
-def test_adjoint_matches_oracle(hip_device, precision, weight, partials):
+@pytest.mark.parametrize("weight,partials,nspecies", [("neutron", False, 2), ("unit", True, 2),
+                                                      ("xray", True, 2), ("xray", False, 2),
+                                                      ("xray", True, 4), ("neutron", True, 4)])
+def test_adjoint_matches_oracle(hip_device, precision, weight, partials, nspecies):
+    """Four species run the x pass and its adjoint with the field count read
+    at run time (the compiled instances hold one and two fields)."""
     import torch
 
     from sqcalc_amd import HipStructureFactor
     from sqcalc_amd.autograd import structure_factor
 
     cell, origin, types, frames = random_case(natoms=300, length=15.0, nframes=1, seed=21,
-                                              tilt=(1.0, -0.5, 0.3))
-    mapping = {1: "Si", 2: "O"}
+                                              tilt=(1.0, -0.5, 0.3), ntypes=nspecies)
+    mapping = MAPPINGS[nspecies]
     qmax, nq = 4.5, 30
-    print("case adjoint", precision, weight, partials, flush=True)
+    print("case adjoint", precision, weight, partials, nspecies, flush=True)
     eps = 1e-11 if precision == "double" else 1e-6
     sf = HipStructureFactor(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax,
                             nq=nq, precision=precision, partials=partials, eps=eps)
@@ -48,7 +54,7 @@ def test_adjoint_matches_oracle(hip_device, precision, weight, partials):
 
     run = O.Run(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax, nq=nq,
                 partials=partials)
-    rho = O.direct_rho(frames[0], cell, origin, run.species_of, 2, run.md, method=0)
+    rho = O.direct_rho(frames[0], cell, origin, run.species_of, nspecies, run.md, method=0)
     num, _ = O.shell_sums(rho, run.md, weight, run.amp, run.xcoef)
     Sref = np.where(run.den > 0, num / np.where(run.den > 0, run.den, 1), 0)
     np.testing.assert_allclose(S.detach().cpu().numpy(), Sref, rtol=0,
@@ -56,9 +62,13 @@ def test_adjoint_matches_oracle(hip_device, precision, weight, partials):
     gS = 2 * (Sref - target.cpu().numpy())
     gP = None
     if partials:
-        gP = np.zeros((2, 2, nq))
+        gP = np.zeros((nspecies, nspecies, nq))
         w = wP.cpu().numpy()
-        gP[0, 0], gP[0, 1], gP[1, 1] = w[0], w[1], w[2]
+        k = 0
+        for a in range(nspecies):
+            for b in range(a, nspecies):
+                gP[a, b] = w[k]
+                k += 1
     gref = O.gradient(frames[0], cell, origin, run.species_of, run.md, rho, weight, run.amp,
                       run.xcoef, gS, run.den, run.shell_count, gP=gP)
     err = np.abs(grad - gref).max() / np.abs(gref).max()
@@ -104,32 +114,44 @@ def test_adjoint_dense_windows(hip_device, precision):
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-def test_loss_step_under_graph_capture(hip_device, precision):
+@pytest.mark.parametrize("partials", [False, True])
+def test_loss_step_under_graph_capture(hip_device, precision, partials):
     """The forward and adjoint entry points order their work on the caller's
     stream with events and make no host synchronisation, so a whole loss step
     (forward, loss, backward) can be captured with torch.cuda.graph and
     replayed per frame.  Each replay, with eager work and a synchronisation
     in between, must give the eager gradient of its frame: 1e-10 relative in
     FP64; in FP32 the FP64 shell sums' atomic order can flip single-precision
-    roundings downstream, so the gradient tolerance applies."""
+    roundings downstream, so the gradient tolerance applies.  With partials
+    the loss adds a weighted sum of the pair columns, so the pair gradients
+    go through the captured backward as well."""
     import torch
 
     from sqcalc_amd import HipStructureFactor
-    from sqcalc_amd.autograd import sq_matching_loss
+    from sqcalc_amd.autograd import sq_matching_loss, structure_factor
 
     cell, origin, types, frames = random_case(natoms=800, length=16.0, nframes=4, seed=5,
                                               tilt=(0.8, 0.0, -0.4))
-    print("case adjoint graph capture", precision, flush=True)
+    print("case adjoint graph capture", precision, partials, flush=True)
     eps = 1e-11 if precision == "double" else 1e-6
     sf = HipStructureFactor(cell, origin, types, {1: "Si", 2: "O"}, weight="neutron",
-                            norm="mean", qmax=4.0, nq=40, precision=precision, partials=False,
+                            norm="mean", qmax=4.0, nq=40, precision=precision, partials=partials,
                             eps=eps, tune=True)
-    target = torch.tensor(np.random.default_rng(1).random(40), device=hip_device)
+    rng = np.random.default_rng(1)
+    target = torch.tensor(rng.random(40), device=hip_device)
+    wP = torch.tensor(rng.standard_normal((3, 40)), device=hip_device)
+
+    def loss_of(x):
+        if not partials:
+            return sq_matching_loss(x, sf, target)
+        S, P = structure_factor(x, sf)
+        return torch.sum((S - target) ** 2) + torch.sum(wP * P)
+
     pos = torch.tensor(frames, device=hip_device)
     ref = []
     for k in range(len(frames)):
         x = pos[k].clone().requires_grad_(True)
-        sq_matching_loss(x, sf, target).backward()
+        loss_of(x).backward()
         ref.append(x.grad.clone())
     static = pos[0].clone().requires_grad_(True)
     side = torch.cuda.Stream()
@@ -137,13 +159,13 @@ def test_loss_step_under_graph_capture(hip_device, precision):
     with torch.cuda.stream(side):
         for _ in range(2):
             static.grad = None
-            sq_matching_loss(static, sf, target).backward()
+            loss_of(static).backward()
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     static.grad = None
     with torch.cuda.graph(graph):
-        sq_matching_loss(static, sf, target).backward()
+        loss_of(static).backward()
     tol = 1e-10 if precision == "double" else TOL["single"]
     for rep in range(3):
         for k in range(len(frames)):

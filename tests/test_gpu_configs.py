@@ -15,13 +15,17 @@ c2  L=113, N=96,000, neutron, partials, q_max 10:
     - the perfect 46^3 lattice: S of a shell is N * (Bragg modes in the
       shell) / (modes in the shell), every other mode giving exactly zero.
 c3  L=80, N=32,000, 60 frames: graph replays against kernel-by-kernel
-    launches (1e-12).
+    launches (1e-12); the autotuned graph plan over 100 frames against the
+    oracle on its 25 lowest shells (1e-6).
 c4  L=44, N=8,000, 3 species: dL/dr of sum_s (S_s - T_s)^2 against the
     oracle's analytic gradient (1e-8 FP64, 1e-5 FP32 at eps 1e-6) and against
     central differences of the GPU FP64 forward on 64 coordinates (h = 1e-4 A).
 c5  triclinic 250 A, N = 10^6, 4 species, xray, 10 partials, q_max 4: FP32
     against FP64 on every column, 1e-6; the FP64 plan against the oracle's
-    direct sum on the 25 shells below q = 0.2 (dq = 0.008 in both), 1e-9.
+    direct sum on the 25 shells below q = 0.2 (dq = 0.008 in both), 1e-9;
+    dL/dr of a loss on those shells (S and all ten pair columns) against the
+    oracle's analytic gradient for every atom (1e-8 FP64, 1e-5 FP32), and
+    the FP32 gradient of the full-range loss against the FP64 one (1e-5).
 Tolerances use the reference suite's metric max |d| / max(|ref|, 1).
 """
 
@@ -169,6 +173,34 @@ def test_c3_graph_replay(hip_device):
                                atol=1e-12 * np.abs(res[0][1]).max())
 
 
+def test_c3_autotuned_graph_low_shells_against_oracle(hip_device):
+    """The batched-trajectory plan as the benchmark runs it (autotuned tiles,
+    captured graphs of 50 frames, FP32): 100 frames of the random walk, the
+    25 lowest shells (q < 0.4) of every column against the oracle's direct
+    sum over the same frames, 1e-6."""
+    from sqcalc_amd.synthetic import config
+    wl = config("c3")
+    frames = wl.frames(0, 100)
+    print("case c3 tuned graph vs oracle", flush=True)
+    sf = _plan(wl, "single", tune=True, max_batch=50)
+    sf.accumulate_device(_device(frames))
+    got = sf.table()
+    _, _, _, nfr = sf.fetch()
+    assert nfr == 100
+    sf.close()
+    nlow = 25
+    dq = (wl.qmax - wl.qmin) / wl.nq
+    run = O.Run(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight, norm=wl.norm,
+                qmin=wl.qmin, qmax=wl.qmin + nlow * dq, nq=nlow)
+    for f in frames:
+        run.add_frame(f)
+    ref = run.table()
+    np.testing.assert_allclose(got[:nlow, 0], ref[:, 0], atol=1e-12)
+    dev = rel_dev(ref[:, 1:], got[:nlow, 1:])
+    print("  fp32 graph vs oracle on %d shells: %.2e" % (nlow, dev))
+    assert dev < 1e-6
+
+
 @pytest.fixture(scope="module")
 def c4_case():
     from sqcalc_amd.synthetic import config
@@ -260,3 +292,91 @@ def test_c5_single_against_double(hip_device):
     dev = rel_dev(ref[:, 1:], full[:nlow, 1:])
     print("  fp64 vs oracle on %d shells: %.2e" % (nlow, dev))
     assert dev < 1e-9
+
+
+@pytest.fixture(scope="module")
+def c5_adjoint_case():
+    """c5's first frame, the direct-sum oracle on the 25 lowest shells
+    (q < 0.2, about 2,100 modes) and the oracle's analytic gradient of
+        L = sum_{s < 25} (S_s - T_s)^2 + sum_{pairs, s < 25} wP_ab,s P_ab,s
+    with T = S_ref / 2 and a seeded random pair weight wP (0 on every other
+    shell), so every one of the 10^6 atoms gets a gradient from all four
+    species and all ten pairs."""
+    from sqcalc_amd.synthetic import config
+    wl = config("c5")
+    frame = wl.frames(0, 1)[0]
+    nlow = 25
+    dq = (wl.qmax - wl.qmin) / wl.nq
+    run = O.Run(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight, norm=wl.norm,
+                qmin=wl.qmin, qmax=wl.qmin + nlow * dq, nq=nlow)
+    S = len(run.species_type)
+    rho = O.direct_rho(frame, wl.cell, wl.origin, run.species_of, S, run.md, method=1)
+    num, _ = O.shell_sums(rho, run.md, wl.weight, run.amp, run.xcoef)
+    Sref = np.where(run.den > 0, num / np.where(run.den > 0, run.den, 1), 0)
+    T = 0.5 * Sref
+    wP = np.random.default_rng(55).standard_normal((S * (S + 1) // 2, nlow))
+    gP = np.zeros((S, S, nlow))
+    k = 0
+    for a in range(S):
+        for b in range(a, S):
+            gP[a, b] = wP[k]
+            k += 1
+    gref = O.gradient(frame, wl.cell, wl.origin, run.species_of, run.md, rho, wl.weight,
+                      run.amp, run.xcoef, 2 * (Sref - T), run.den, run.shell_count, gP=gP)
+    return wl, frame, nlow, T, wP, gref
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-8), ("single", 1e-5)])
+def test_c5_adjoint_low_shells_against_oracle(hip_device, c5_adjoint_case, precision, tol):
+    """The forward + adjoint of the 10^6-atom triclinic X-ray workload with
+    its four fields and ten pair columns (the x pass and its adjoint with
+    the field count read at run time) against the oracle's gradient, every
+    atom: max |g - g_ref| / max |g_ref|.  The plan covers the whole q range;
+    the loss reads its 25 lowest shells."""
+    import torch
+
+    from sqcalc_amd.autograd import structure_factor
+    wl, frame, nlow, T, wP, gref = c5_adjoint_case
+    print("case c5 adjoint", precision, flush=True)
+    eps = 1e-11 if precision == "double" else 1e-6
+    sf = _plan(wl, precision, eps=eps)
+    assert sf.info.nfields == 4 and sf.info.ncomp == 11
+    pos = torch.tensor(frame, device=hip_device, requires_grad=True)
+    S, P = structure_factor(pos, sf)
+    Tt = torch.tensor(T, device=hip_device)
+    wPt = torch.tensor(wP, device=hip_device)
+    loss = torch.sum((S[:nlow] - Tt) ** 2) + torch.sum(wPt * P[:, :nlow])
+    loss.backward()
+    g = pos.grad.cpu().numpy()
+    err = np.abs(g - gref).max() / np.abs(gref).max()
+    print("  adjoint vs analytic on %d atoms: %.2e" % (len(frame), err))
+    assert err < tol
+    sf.close()
+
+
+def test_c5_adjoint_single_against_double_full_range(hip_device, c5_adjoint_case):
+    """The S(q)-matching loss over all 500 shells with the pair columns:
+    the FP32 adjoint (eps 1e-6) against the FP64 one, 1e-5 relative."""
+    import torch
+
+    from sqcalc_amd.autograd import structure_factor
+    wl, frame, _, _, _, _ = c5_adjoint_case
+    print("case c5 adjoint fp32 vs fp64", flush=True)
+    grads = {}
+    target = None
+    wP = None
+    for precision, eps in (("double", 1e-11), ("single", 1e-6)):
+        sf = _plan(wl, precision, eps=eps)
+        pos = torch.tensor(frame, device=hip_device, requires_grad=True)
+        S, P = structure_factor(pos, sf)
+        if target is None:
+            target = 0.5 * S.detach().clone()
+            wP = torch.tensor(np.random.default_rng(56).standard_normal(tuple(P.shape)),
+                              device=hip_device)
+        loss = torch.sum((S - target) ** 2) + torch.sum(wP * P)
+        loss.backward()
+        grads[precision] = pos.grad.cpu().numpy()
+        sf.close()
+    err = np.abs(grads["single"] - grads["double"]).max() / np.abs(grads["double"]).max()
+    print("  fp32 vs fp64 gradient %.2e" % err)
+    assert err < 1e-5

@@ -127,23 +127,57 @@ def test_dense_windows_span_many_staging_rounds(hip_device, precision):
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-def test_eight_species_shell_sums_in_hbm(hip_device, precision):
-    """8 species with partials at 500 shells: 37 rows of shell sums (148 KB
-    of FP64) leave no LDS for even one x line, so the x pass keeps them in
-    its rows of the per-workgroup partials in HBM.  Every column against the
-    oracle (eps 1e-10 in FP64 and 1e-6 in FP32: with 60 atoms a species the
-    pair columns are small, and this case checks the accumulation path, not
-    the FP32 default's accuracy, which the golden tables pin)."""
+@pytest.mark.parametrize("nq,in_hbm", [(500, 0), (600, 1)])
+def test_eight_species_shell_sums(hip_device, precision, nq, in_hbm):
+    """8 species with partials: 37 rows of FP64 shell sums.  At 500 shells
+    (148 KB) they still fit the LDS beside one x line; at 600 shells
+    (178 KB) they do not, and the x pass adds them in its own row of the
+    per-workgroup partials in HBM (the plan reports which).  Every column
+    against the oracle (eps 1e-10 in FP64 and 1e-6 in FP32: with 60 atoms a
+    species the pair columns are small, and this case checks the
+    accumulation path, not the FP32 default's accuracy, which the golden
+    tables pin)."""
     cell, origin, types, frames = random_case(natoms=480, length=12.0, nframes=2, seed=17,
                                               ntypes=8, tilt=(0.5, 0.0, -0.3))
-    print("case eight species", precision, flush=True)
-    sf = _hip(cell, origin, types, None, "unit", "mean", 0.0, 4.0, 500, precision,
+    print("case eight species", precision, nq, flush=True)
+    qmax = 4.0 * nq / 500.0
+    sf = _hip(cell, origin, types, None, "unit", "mean", 0.0, qmax, nq, precision,
               eps=1e-10 if precision == "double" else 1e-6)
+    assert sf.info.shell_sums_in_hbm == in_hbm
     sf.accumulate(frames)
     table = sf.table()
     assert table.shape[1] == 2 + 36
     sf.close()
-    ref = oracle_run(cell, origin, types, None, "unit", "mean", 0.0, 4.0, 500, frames).table()
+    ref = oracle_run(cell, origin, types, None, "unit", "mean", 0.0, qmax, nq, frames).table()
     devs = [rel_dev(ref[:, c], table[:, c]) for c in range(1, ref.shape[1])]
     print("  worst column %.2e" % max(devs))
     assert max(devs) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_faber_ziman_against_oracle_and_high_q_limits(hip_device, precision):
+    """Faber-Ziman columns A_ab = (S_ab - x_a d_ab) / (x_a x_b) + 1 against the
+    oracle's, and the closed-form limits of an ideal 50/50 mixture that the
+    reference suite checks above q = 3 (test/test_sqcalc.py:774-783): the
+    OVITO S_aa -> x_a = 0.5, S_ab -> 0 and A_ab -> 1 (within 0.05, as there)."""
+    cell, origin, types, frames = random_case(natoms=2000, length=30.0, nframes=3, seed=74)
+    print("case faber-ziman", precision, flush=True)
+    args = (cell, origin, types, None, "unit", "n", 0.5, 5.9, 27)
+    ref = oracle_run(*args, frames, faber_ziman=True).table()
+    eps = 1e-10 if precision == "double" else 1e-6
+    sf = _hip(*args, precision, eps=eps, faber_ziman=True)
+    sf.accumulate(frames)
+    fz = sf.table()
+    sf.close()
+    sp = _hip(*args, precision, eps=eps)
+    sp.accumulate(frames)
+    part = sp.table()
+    sp.close()
+    assert fz.shape == ref.shape == (27, 5)
+    dev = rel_dev(ref[:, 1:], fz[:, 1:])
+    print("  A_ab vs oracle: %.2e" % dev)
+    assert dev < TOL[precision]
+    high = part[:, 0] > 3.0
+    saa, sab, aab = part[high, 2].mean(), np.abs(part[high, 3]).mean(), fz[high, 3].mean()
+    print("  high q: S_aa %.3f  |S_ab| %.3f  A_ab %.3f" % (saa, sab, aab))
+    assert abs(saa - 0.5) < 0.05 and sab < 0.1 and abs(aab - 1.0) < 0.05
