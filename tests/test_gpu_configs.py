@@ -326,19 +326,26 @@ def c5_adjoint_case():
     return wl, frame, nlow, T, wP, gref
 
 
-@pytest.mark.parametrize("precision,tol", [("double", 1e-8), ("single", 1e-5)])
-def test_c5_adjoint_low_shells_against_oracle(hip_device, c5_adjoint_case, precision, tol):
+@pytest.mark.parametrize("precision,eps", [("double", 1e-13), ("single", 1e-7)])
+def test_c5_adjoint_low_shells_against_oracle(hip_device, c5_adjoint_case, precision, eps):
     """The forward + adjoint of the 10^6-atom triclinic X-ray workload with
     its four fields and ten pair columns (the x pass and its adjoint with
     the field count read at run time) against the oracle's gradient, every
-    atom: max |g - g_ref| / max |g_ref|.  The plan covers the whole q range;
-    the loss reads its 25 lowest shells."""
+    atom: max |g - g_ref| / max |g_ref| below 1e-8 (FP64) and 1e-5 (FP32).
+    The plan covers the whole q range; the loss reads its 25 lowest shells.
+
+    A loss confined to q < 0.2 is the hard case for the kernel-derivative
+    gather: the gradient is of order q |V| while the derivative's aliasing
+    error is of order eps |V| / h (h the grid spacing), about 10^4 eps
+    relative here (measured in FP64: 1.6e-7 at eps 1e-11, 1.6e-9 at 1e-13;
+    in FP32: 3.0e-4 at 1e-6, 8.3e-6 at 1e-7, where ns = 12).  Hence the
+    tolerances of the gradient plans: eps 1e-13 in FP64, 1e-7 in FP32 (the
+    library clips it to 1.2e-7 and reports the warning code)."""
     import torch
 
     from sqcalc_amd.autograd import structure_factor
     wl, frame, nlow, T, wP, gref = c5_adjoint_case
     print("case c5 adjoint", precision, flush=True)
-    eps = 1e-11 if precision == "double" else 1e-6
     sf = _plan(wl, precision, eps=eps)
     assert sf.info.nfields == 4 and sf.info.ncomp == 11
     pos = torch.tensor(frame, device=hip_device, requires_grad=True)
@@ -350,13 +357,16 @@ def test_c5_adjoint_low_shells_against_oracle(hip_device, c5_adjoint_case, preci
     g = pos.grad.cpu().numpy()
     err = np.abs(g - gref).max() / np.abs(gref).max()
     print("  adjoint vs analytic on %d atoms: %.2e" % (len(frame), err))
-    assert err < tol
+    assert err < (1e-8 if precision == "double" else 1e-5)
     sf.close()
 
 
 def test_c5_adjoint_single_against_double_full_range(hip_device, c5_adjoint_case):
     """The S(q)-matching loss over all 500 shells with the pair columns:
-    the FP32 adjoint (eps 1e-6) against the FP64 one, 1e-5 relative."""
+    the FP32 adjoint (eps 1e-7, ns = 12) against the FP64 one (eps 1e-13),
+    1e-5 relative.  The Bragg shells of the jittered lattice make this the
+    harder case for FP32: measured 1.5e-3 at eps 1e-5, 4.6e-5 at 1e-6,
+    8.0e-7 at 1e-7."""
     import torch
 
     from sqcalc_amd.autograd import structure_factor
@@ -365,7 +375,7 @@ def test_c5_adjoint_single_against_double_full_range(hip_device, c5_adjoint_case
     grads = {}
     target = None
     wP = None
-    for precision, eps in (("double", 1e-11), ("single", 1e-6)):
+    for precision, eps in (("double", 1e-13), ("single", 1e-7)):
         sf = _plan(wl, precision, eps=eps)
         pos = torch.tensor(frame, device=hip_device, requires_grad=True)
         S, P = structure_factor(pos, sf)
