@@ -174,6 +174,24 @@ int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const v
                            const double *gS_device, const double *gP_device,
                            double *grad_device, void *stream);
 
+/* sqc_hip_adjoint_device for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  No counterpart in the reference.  Same arguments and result;
+ * allocates nothing.  The kernels' gradient slots (3 doubles per (atom,
+ * tile) pair: natoms x sx x sy x 24 bytes, sx x sy = 28 at the 8 x 2 spread
+ * tile and ns = 10) live in a static device buffer of
+ * SQC_HIP_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends,
+ * instead of the buffer sqc_hip_adjoint_device allocates on its first call.
+ * A plan whose slots exceed it (about 150,000 atoms at that tile) gets
+ * SQC_HIP_ERR_MAXNALLOC with the sizes in the message.  Every plan on a GPU
+ * shares the buffer and it keeps the last call's contents (nothing in it
+ * needs to start at zero): calls that overlap on one GPU, from other host
+ * threads or on streams not ordered with each other, race. */
+#define SQC_HIP_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+int sqc_hip_adjoint_device_noalloc(sqc_hip_plan *plan, const double *pos_device,
+                                   const void *rho_device, const double *gS_device,
+                                   const double *gP_device, double *grad_device, void *stream);
+
 /* Convenience: loss and gradient of L = sum_s w_s (S_s - T_s)^2 for one
  * frame, host memory in and out (target[nq], wq[nq], grad[3*natoms]). */
 int sqc_hip_gradient(sqc_hip_plan *plan, const double *pos, const double *target,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "sqc_hip_reset": (_c.c_int, [_vp]),
     "sqc_hip_forward_device": (_c.c_int, [_vp, _vp, _c.c_int64, _vp, _vp, _vp]),
     "sqc_hip_adjoint_device": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sqc_hip_adjoint_device_noalloc": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sqc_hip_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sqc_hip_profile": (_c.c_int, [_vp, _c.c_int]),
     "sqc_hip_stage_times": (_c.c_int, [_vp, _vp, _vp, _c.c_int]),

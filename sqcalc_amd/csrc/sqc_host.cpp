@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,9 +34,21 @@ struct sqc_hip_plan {
   void *g_rho = nullptr;
 };
 
+// Device scratch of sqc_hip_adjoint_device_noalloc: the adjoint's gradient
+// slots (3 doubles per (atom, tile) pair) for callers that cannot allocate.
+// Each GPU the library runs on holds one copy until the process ends, and
+// every call on that GPU shares it.  In host code the name is only a
+// stand-in: kernels get the device address from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[SQC_HIP_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
 namespace {
 
 thread_local std::string g_last_error;
+
+// Device address of `scratch` per device ordinal, looked up once per device
+// when the first plan on that device is made (the device is current then).
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -52,6 +65,24 @@ int hip_fail(hipError_t e, const char *what) {
     hipError_t e_ = (call);                          \
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
+
+// Look up (once per device, with the device current) where this device holds
+// the no-allocation scratch.
+int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return 0;
+  void *ptr = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!ptr) return fail(SQC_HIP_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+  g_scratch_ptr[device] = static_cast<double *>(ptr);
+  return 0;
+}
+
+double *scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 
 double det3(const double *m) {
   auto M = [&](int i, int j) { return m[i + 3 * j]; };
@@ -668,6 +699,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
                                            " device(s) available");
   p.device = cfg->device;
   HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  if (int rc = scratch_setup(p.device)) return rc;
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
   p.ncu = prop.multiProcessorCount;
@@ -1301,7 +1333,35 @@ int sqc_hip_adjoint_device(sqc_hip_plan *plan, const double *pos_device, const v
     if (rc) return rc;
   }
   OrderScope scope(p, stream);
-  const int rc = sqc::launch_adjoint(p, pos_device, rho_device, gS_device, gP_device, grad_device);
+  const int rc = sqc::launch_adjoint(p, pos_device, rho_device, gS_device, gP_device, grad_device,
+                                     p.d_slots);
+  if (rc) return hip_fail((hipError_t)rc, "adjoint launch");
+  return SQC_HIP_OK;
+}
+
+int sqc_hip_adjoint_device_noalloc(sqc_hip_plan *plan, const double *pos_device,
+                                   const void *rho_device, const double *gS_device,
+                                   const double *gP_device, double *grad_device, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!pos_device || !rho_device || !gS_device || !grad_device)
+    return fail(SQC_HIP_ERR_BAD_ARG, "bad adjoint arguments");
+  sqc::Plan &p = plan->p;
+  if (gP_device && p.geo.ncomp == 1)
+    return fail(SQC_HIP_ERR_BAD_ARG, "pair gradients need a plan created with partials");
+  const size_t need = sizeof(double) * 3 * (size_t)p.geo.natoms * p.geo.sx * p.geo.sy;
+  if (need > SQC_HIP_NOALLOC_SCRATCH_BYTES)
+    return fail(SQC_HIP_ERR_MAXNALLOC,
+                "the adjoint's gradient slots need " + std::to_string(need) +
+                    " bytes, more than the " + std::to_string(SQC_HIP_NOALLOC_SCRATCH_BYTES) +
+                    "-byte no-allocation scratch (" + std::to_string(p.geo.natoms) +
+                    " atoms x " + std::to_string(p.geo.sx * p.geo.sy) +
+                    " tiles x 24 B); use sqc_hip_adjoint_device");
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  double *slots = scratch_of(p.device);
+  if (!slots) return fail(SQC_HIP_ERR_HIP, "no scratch address for this device");
+  OrderScope scope(p, stream);
+  const int rc =
+      sqc::launch_adjoint(p, pos_device, rho_device, gS_device, gP_device, grad_device, slots);
   if (rc) return hip_fail((hipError_t)rc, "adjoint launch");
   return SQC_HIP_OK;
 }

@@ -165,7 +165,7 @@ struct Plan {
 int launch_forward(Plan &p, const double *d_pos_frame, int mode, double *frame_out,
                    void *rho_out);
 int launch_adjoint(Plan &p, const double *d_pos_frame, const void *rho, const double *gS,
-                   const double *gP, double *grad);
+                   const double *gP, double *grad, double *slots);
 int launch_forward_front(Plan &p, const double *d_pos_frame, void *W1, hipStream_t st);
 int launch_forward_y(Plan &p, const void *W1, void *W2, hipStream_t st);
 int launch_forward_x(Plan &p, const void *W2, hipStream_t st, int mode, double *frame_out,

@@ -1953,7 +1953,7 @@ int launch_forward(Plan &p, const double *d_pos, int mode, double *frame_out, vo
 
 template <typename T>
 static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double *gS,
-                     const double *gP, double *grad) {
+                     const double *gP, double *grad, double *slots) {
   using V = typename C2<T>::t;
   Geo &g = p.geo;
   hipStream_t st = p.stream;
@@ -1994,7 +1994,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
                            dim3(g.ntx * g.nty, Fa), dim3(NT), p.lds1g, st, g, p.ax[2],
                            (const V *)p.d_W1, (const V *)p.d_tw[2], p.d_i0s,
                            (const T *)p.d_phi, (const T *)p.d_dphi, p.d_sidx, p.d_start,
-                           p.d_slots);
+                           slots);
       };
       if (g.txr == 8 && g.ty == 2)
         go(std::integral_constant<int, 1>{});
@@ -2004,17 +2004,20 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   }
   {
     StageTimer t(p, 8);
-    hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_i0, p.d_species, p.d_slots,
+    hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_i0, p.d_species, slots,
                        Fa, grad);
   }
   SQC_CHECK(hipGetLastError());
   return 0;
 }
 
+// slots: 3 doubles per (atom, tile) pair, [natoms][sx][sy][3]; every slot
+// that k_adj_sum reads is written by k_adj_z_gather in the same launch
+// sequence, so the buffer needs no clearing.
 int launch_adjoint(Plan &p, const double *d_pos, const void *rho, const double *gS,
-                   const double *gP, double *grad) {
-  return p.prec == 1 ? adjoint_t<float>(p, d_pos, rho, gS, gP, grad)
-                     : adjoint_t<double>(p, d_pos, rho, gS, gP, grad);
+                   const double *gP, double *grad, double *slots) {
+  return p.prec == 1 ? adjoint_t<float>(p, d_pos, rho, gS, gP, grad, slots)
+                     : adjoint_t<double>(p, d_pos, rho, gS, gP, grad, slots);
 }
 
 // Raise the dynamic-LDS ceiling of each kernel to what the plan needs.
