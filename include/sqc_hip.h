@@ -48,7 +48,9 @@ enum {
   SQC_HIP_ERR_ABI = 30,             /* config struct from another ABI version */
   SQC_HIP_ERR_NO_DEVICE = 31,       /* no usable HIP device */
   SQC_HIP_ERR_HIP = 32,             /* a HIP runtime call failed */
-  SQC_HIP_ERR_NO_MODES = 33         /* no reciprocal lattice point in [qmin, qmax] */
+  SQC_HIP_ERR_NO_MODES = 33,        /* no reciprocal lattice point in [qmin, qmax] */
+  SQC_HIP_ERR_UNKNOWN_ELEMENT = 34, /* species label names no element */
+  SQC_HIP_ERR_NO_CHARGE_STATE = 35  /* element known, charge or valence state not tabulated */
 };
 
 enum { SQC_HIP_F64 = 0, SQC_HIP_F32 = 1 };
@@ -215,6 +217,42 @@ const char *sqc_hip_last_error(void);
 /* sizeof(sqc_hip_config), for the host-side layout check that mirrors the
  * reference's opts ABI check (src/sqc_cufinufft.f90:277-290). */
 size_t sqc_hip_config_size(void);
+
+/* ---- Species labels and their scattering data (host side, no device) ----
+ *
+ * The -m TYPE:LABEL mapping resolves each label to an element and, for the
+ * X-ray weights, to a row of the IT92 table: a neutral atom ("Si"), an ion
+ * ("O2-", "Na1+", "Fe+") or a valence state ("Cval").  Replaces
+ * element_table_t%find_species (src/sqc_elements.f90:241-291) with the data
+ * of src/sqc_element_data.f90 (104 elements, 213 X-ray species, 2 aliases);
+ * the amplitudes follow weight_scheme_t%amplitude (src/sqc_weights.f90:
+ * 136-154): b of the element under neutron weights, f(q) = c + sum_i a_i
+ * exp(-b_i (q / 4 pi)^2) of the label under X-ray weights. */
+typedef struct sqc_hip_species_info {
+  char label[16];           /* canonical label ("Si", "O2-", "Na1+", "Cval") */
+  char symbol[4];           /* element symbol */
+  int32_t z;                /* atomic number */
+  int32_t charge;           /* formal charge (0 for neutral atoms and valence states) */
+  int32_t valence;          /* 1 for a valence-state row */
+  int32_t has_neutron;      /* 1: the element's bound coherent length is tabulated */
+  int32_t has_xray;         /* 1: xray_coef holds the label's IT92 row */
+  int32_t reserved;
+  double neutron_b;         /* fm, of the element (ions take their element's) */
+  double xray_coef[9];      /* a1..a4, b1..b4, c (config xray_coef layout) */
+} sqc_hip_species_info;
+
+/* Resolve a label as table_find_species does: SQC_HIP_OK (a bare symbol of
+ * an element without an X-ray row resolves with has_xray = 0),
+ * SQC_HIP_ERR_UNKNOWN_ELEMENT or SQC_HIP_ERR_NO_CHARGE_STATE, with the
+ * reference's message (listing the tabulated states) in
+ * sqc_hip_species_error(). */
+int sqc_hip_species_lookup(const char *label, sqc_hip_species_info *info);
+/* Row `index` of the element table (kind 0: each element with its neutral
+ * X-ray row when it has one) or of the X-ray species table (kind 1);
+ * *count (may be NULL) gets the table's length, info may be NULL. */
+int sqc_hip_species_table(int32_t kind, int32_t index, sqc_hip_species_info *info,
+                          int32_t *count);
+const char *sqc_hip_species_error(void);
 
 /* ---- LAMMPS dump reader (host side, no device needed) ----
  *

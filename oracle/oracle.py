@@ -29,24 +29,26 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 # IT92 X-ray coefficients (a1..a4, b1..b4, c) and bound coherent neutron
-# lengths b [fm] of the species the tests and benchmarks use.  Values are the
-# published tables the reference carries (src/sqc_element_data.f90; IT92
-# 6.1.1.4, Neutron News 3 (1992) 29-37).
-NEUTRON_B = {"H": -3.739, "C": 6.646, "O": 5.803, "Na": 3.63, "Al": 3.449, "Si": 4.1491,
-             "Cl": 9.577, "Ca": 4.7, "Ni": 10.3}
-XRAY = {
-    "H": ([0.493002, 0.322912, 0.140191, 0.04081], [10.5109, 26.1257, 3.14236, 57.7997], 0.003038),
-    "C": ([2.31, 1.02, 1.5886, 0.865], [20.8439, 10.2075, 0.5687, 51.6512], 0.2156),
-    "O": ([3.0485, 2.2868, 1.5463, 0.867], [13.2771, 5.7011, 0.3239, 32.9089], 0.2508),
-    "Na": ([4.7626, 3.1736, 1.2674, 1.1128], [3.285, 8.8422, 0.3136, 129.424], 0.676),
-    "Al": ([6.4202, 1.9002, 1.5936, 1.9646], [3.0387, 0.7426, 31.5472, 85.0886], 1.1151),
-    "Si": ([6.2915, 3.0353, 1.9891, 1.541], [2.4386, 32.3337, 0.6785, 81.6937], 1.1407),
-    "Cl": ([11.4604, 7.1964, 6.2556, 1.6455], [0.0104, 1.1662, 18.5194, 47.7784], -9.5574),
-    "Ca": ([8.6266, 7.3873, 1.5899, 1.0211], [10.4421, 0.6599, 85.7484, 178.437], 1.3751),
-    "Ni": ([12.8376, 7.292, 4.4438, 2.38], [3.8785, 0.2565, 12.1763, 66.3421], 1.0341),
-    "O2-": ([3.7504, 2.84294, 1.54298, 1.62091], [16.5151, 6.59203, 0.319201, 43.3486], 0.24206),
-    "Si4+": ([4.43918, 3.20345, 1.19453, 0.41653], [1.64167, 3.43757, 0.2149, 6.65365], 0.746297),
-}
+# lengths b [fm]: the reference's own tables (src/sqc_element_data.f90; IT92
+# 6.1.1.4, Neutron News 3 (1992) 29-37) as the fixture
+# tests/golden/elements/elements.json holds them (tests/golden/make_elements_golden.py).
+_TABLES = None
+
+
+def element_tables():
+    """(elements, species) rows of the fixture: elements by symbol, species
+    by canonical label."""
+    global _TABLES
+    if _TABLES is None:
+        import json
+        path = os.path.join(HERE, "..", "tests", "golden", "elements", "elements.json")
+        with open(path) as fh:
+            d = json.load(fh)
+        _TABLES = ({e["symbol"]: e for e in d["elements"]},
+                   {sp["label"]: sp for sp in d["species"]}, d)
+    return _TABLES
+
+
 WEIGHT_KIND = {"unit": 0, "neutron": 1, "xray": 2}
 NORM_KIND = {"mean": 1, "self": 2, "n": 3}
 
@@ -148,13 +150,17 @@ def weight_params(kind, labels):
     S = len(labels)
     amp = np.ones(S)
     xcoef = np.zeros((S, 9))
+    elements, species, d = element_tables()
     for i, lab in enumerate(labels):
         if kind == "neutron":
-            elem = lab.rstrip("+-0123456789").replace("val", "")
-            amp[i] = NEUTRON_B[elem]
+            if lab in species:
+                e = d["elements"][species[lab]["element"] - 1]
+            else:
+                e = elements[lab]
+            assert e["has_neutron"], lab
+            amp[i] = e["neutron_b"]
         elif kind == "xray":
-            a, b, c = XRAY[lab]
-            xcoef[i] = list(a) + list(b) + [c]
+            xcoef[i] = species[lab]["coef"]
     return amp, xcoef
 
 

@@ -47,6 +47,15 @@ class Info(_c.Structure):
     ]
 
 
+class SpeciesInfo(_c.Structure):
+    _fields_ = [
+        ("label", _c.c_char * 16), ("symbol", _c.c_char * 4), ("z", _c.c_int32),
+        ("charge", _c.c_int32), ("valence", _c.c_int32), ("has_neutron", _c.c_int32),
+        ("has_xray", _c.c_int32), ("reserved", _c.c_int32), ("neutron_b", _c.c_double),
+        ("xray_coef", _c.c_double * 9),
+    ]
+
+
 class DumpFrame(_c.Structure):
     _fields_ = [
         ("natoms", _c.c_int64), ("timestep", _c.c_int64), ("cell", _c.c_double * 9),
@@ -79,6 +88,10 @@ SIGNATURES = {
     "sqc_hip_error_string": (_c.c_char_p, [_c.c_int]),
     "sqc_hip_last_error": (_c.c_char_p, []),
     "sqc_hip_config_size": (_c.c_size_t, []),
+    "sqc_hip_species_lookup": (_c.c_int, [_c.c_char_p, _c.POINTER(SpeciesInfo)]),
+    "sqc_hip_species_table": (_c.c_int, [_c.c_int32, _c.c_int32, _c.POINTER(SpeciesInfo),
+                                         _c.POINTER(_c.c_int32)]),
+    "sqc_hip_species_error": (_c.c_char_p, []),
     "sqc_dump_open": (_c.c_int, [_c.POINTER(_vp), _c.c_char_p, _c.c_int]),
     "sqc_dump_next": (_c.c_int, [_vp, _c.POINTER(DumpFrame)]),
     "sqc_dump_copy": (_c.c_int, [_vp, _vp, _vp, _vp]),

@@ -32,6 +32,7 @@ def build_library(verbose=False, jobs=2):
         ("sqc_host.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off"]
          + DEVICE_FLAGS),
         ("sqc_dump.cpp", ["-x", "c++", "-ffp-contract=off", "-pthread"]),
+        ("sqc_elements.cpp", ["-x", "c++"]),
     ]
     procs = []
     for src, extra in units:

@@ -1,60 +1,77 @@
-"""Scattering data for the species the path is run with.
+"""Species labels of the -m TYPE:LABEL mapping and their scattering data.
 
-Neutron bound coherent lengths b [fm] (Neutron News 3 (1992) 29-37) per
-element and IT92 X-ray form-factor coefficients (International Tables C
-(1992) 6.1.1.4: f(q) = c + sum_i a_i exp(-b_i (q/4pi)^2)) per species label.
-These are the published values the reference tabulates in
-src/sqc_element_data.f90 (amplitude lookup: src/sqc_weights.f90:136-154,
-src/sqc_elements.f90:181-239).  An ion or valence label ("O2-", "Sival")
-uses the neutron length of its element.
+A thin layer over the library's lookup (sqc_hip_species_lookup in
+include/sqc_hip.h), which restates the reference's rules for labels
+(src/sqc_elements.f90:241-411: aliases, case, ions "O2-" / "Na1+" / "Fe+",
+valence states "Cval") over the data of src/sqc_element_data.f90: 104
+elements with their bound coherent neutron lengths b [fm] (Neutron News 3
+(1992) 29-37) and 213 X-ray species with IT92 form-factor coefficients
+(International Tables C (1992) 6.1.1.4, f(q) = c + sum_i a_i exp(-b_i
+(q / 4 pi)^2)).  An ion or valence label takes its element's neutron length
+(src/sqc_weights.f90:136-170).
 """
 
-import re
+import ctypes
 
-NEUTRON_B = {
-    "H": -3.739, "C": 6.646, "O": 5.803, "Na": 3.63, "Al": 3.449, "Si": 4.1491,
-    "Cl": 9.577, "Ca": 4.7, "Ni": 10.3,
-}
-
-# label: (a1..a4, b1..b4, c)
-XRAY = {
-    "H": (0.493002, 0.322912, 0.140191, 0.04081, 10.5109, 26.1257, 3.14236, 57.7997, 0.003038),
-    "C": (2.31, 1.02, 1.5886, 0.865, 20.8439, 10.2075, 0.5687, 51.6512, 0.2156),
-    "O": (3.0485, 2.2868, 1.5463, 0.867, 13.2771, 5.7011, 0.3239, 32.9089, 0.2508),
-    "Na": (4.7626, 3.1736, 1.2674, 1.1128, 3.285, 8.8422, 0.3136, 129.424, 0.676),
-    "Al": (6.4202, 1.9002, 1.5936, 1.9646, 3.0387, 0.7426, 31.5472, 85.0886, 1.1151),
-    "Si": (6.2915, 3.0353, 1.9891, 1.541, 2.4386, 32.3337, 0.6785, 81.6937, 1.1407),
-    "Cl": (11.4604, 7.1964, 6.2556, 1.6455, 0.0104, 1.1662, 18.5194, 47.7784, -9.5574),
-    "Ca": (8.6266, 7.3873, 1.5899, 1.0211, 10.4421, 0.6599, 85.7484, 178.437, 1.3751),
-    "Ni": (12.8376, 7.292, 4.4438, 2.38, 3.8785, 0.2565, 12.1763, 66.3421, 1.0341),
-    "O2-": (3.7504, 2.84294, 1.54298, 1.62091, 16.5151, 6.59203, 0.319201, 43.3486, 0.24206),
-    "Si4+": (4.43918, 3.20345, 1.19453, 0.41653, 1.64167, 3.43757, 0.2149, 6.65365, 0.746297),
-    "Na1+": (3.2565, 3.9362, 1.3998, 1.0032, 2.6671, 6.1153, 0.2001, 14.039, 0.404),
-    "Ca2+": (15.6348, 7.9518, 8.4372, 0.8537, -0.0074, 0.6089, 10.3116, 25.9905, -14.875),
-    "Ni2+": (11.4166, 7.4005, 5.3442, 0.9773, 3.6766, 0.2449, 8.873, 22.1626, 0.8614),
-    "Al3+": (4.17448, 3.3876, 1.20296, 0.528137, 1.93816, 4.14553, 0.228753, 8.28524, 0.706786),
-}
+from . import _lib
 
 
-def element_of(label):
-    m = re.match(r"[A-Z][a-z]?", label)
-    if not m:
-        raise ValueError("not a species label: %r" % label)
-    return m.group(0)
+class SpeciesError(ValueError):
+    """An unresolvable label; code 34 unknown element, 35 no such charge state."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def lookup(label):
+    """dict(label, symbol, z, charge, valence, has_neutron, has_xray,
+    neutron_b, xray_coef) of a species label, or SpeciesError."""
+    lib = _lib.load()
+    info = _lib.SpeciesInfo()
+    rc = lib.sqc_hip_species_lookup(str(label).encode(), info)
+    if rc != 0:
+        raise SpeciesError(rc, lib.sqc_hip_species_error().decode())
+    return _as_dict(info)
+
+
+def _as_dict(info):
+    return {"label": info.label.decode(), "symbol": info.symbol.decode(), "z": info.z,
+            "charge": info.charge, "valence": bool(info.valence),
+            "has_neutron": bool(info.has_neutron), "has_xray": bool(info.has_xray),
+            "neutron_b": info.neutron_b, "xray_coef": tuple(info.xray_coef)}
+
+
+def table(kind):
+    """Every row of the element table ("elements": each element with its
+    neutral X-ray row) or of the X-ray species table ("species")."""
+    lib = _lib.load()
+    k = {"elements": 0, "species": 1}[kind]
+    n = ctypes.c_int32(0)
+    _lib.check(lib.sqc_hip_species_table(k, 0, None, ctypes.byref(n)), "species_table")
+    out = []
+    for i in range(n.value):
+        info = _lib.SpeciesInfo()
+        _lib.check(lib.sqc_hip_species_table(k, i, info, None), "species_table")
+        out.append(_as_dict(info))
+    return out
+
+
+def canonical(label):
+    return lookup(label)["label"]
 
 
 def neutron_b(label):
-    elem = element_of(label)
-    if elem not in NEUTRON_B:
-        raise KeyError("no neutron scattering length for %s (available: %s)"
-                       % (elem, ", ".join(sorted(NEUTRON_B))))
-    return NEUTRON_B[elem]
+    s = lookup(label)
+    if not s["has_neutron"]:
+        raise SpeciesError(15, "no neutron scattering length for %s; use --weight xray or unit"
+                           % s["label"])
+    return s["neutron_b"]
 
 
 def xray_coefficients(label):
-    if label not in XRAY:
-        elem = element_of(label)
-        alt = sorted(k for k in XRAY if element_of(k) == elem)
-        raise KeyError("no X-ray form factor for %s (available: %s)"
-                       % (label, ", ".join(alt) if alt else "none"))
-    return XRAY[label]
+    s = lookup(label)
+    if not s["has_xray"]:
+        raise SpeciesError(15, "no X-ray form factor for %s; use --weight neutron or unit"
+                           % s["label"])
+    return s["xray_coef"]

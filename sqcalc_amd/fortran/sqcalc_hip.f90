@@ -258,28 +258,37 @@ contains
       end do
    end subroutine classify
 
+   !> Amplitudes of the species present (validate_mapping and
+   !! weight_scheme_t%amplitude, app/main.f90:515-551, src/sqc_weights.f90:136-154).
    subroutine species_weights()
-      integer :: k
-      logical :: ok
+      integer :: k, ierr
+      character(len=:), allocatable :: message
+      type(sqc_hip_species_info_t) :: sinfo
       allocate (amp(nspec), xcoef(9, nspec))
       amp = 1.0_rk
       xcoef = 0.0_rk
       do k = 1, nspec
          if (weight_kind == sqc_hip_weight_unit) cycle
          if (len_trim(spec_label(k)) == 0) then
-            write (error_unit, '(a,i0,a)') 'sqcalc_hip: type ', spec_type(k), &
-               ' has no species in the mapping (-m)'
-            stop 6
+            write (error_unit, '(a,i0,a)') 'sqcalc_hip: atom type ', spec_type(k), &
+               ' appears in the dump but is missing from the element mapping'
+            stop 14
          end if
+         call resolve_species(spec_label(k), sinfo, ierr, message)
          if (weight_kind == sqc_hip_weight_neutron) then
-            call neutron_b(trim(spec_label(k)), amp(k), ok)
+            if (sinfo%has_neutron == 0) then
+               write (error_unit, '(a)') 'sqcalc_hip: no neutron scattering length for '// &
+                  trim(spec_label(k))//'; use --weight xray or unit'
+               stop 15
+            end if
+            amp(k) = sinfo%neutron_b
          else
-            call xray_coefficients(trim(spec_label(k)), xcoef(:, k), ok)
-         end if
-         if (.not. ok) then
-            write (error_unit, '(a)') 'sqcalc_hip: no scattering data for "'// &
-               trim(spec_label(k))//'"'
-            stop 6
+            if (sinfo%has_xray == 0) then
+               write (error_unit, '(a)') 'sqcalc_hip: no X-ray form factor for '// &
+                  trim(spec_label(k))//'; use --weight neutron or unit'
+               stop 15
+            end if
+            xcoef(:, k) = sinfo%xray_coef
          end if
       end do
    end subroutine species_weights
@@ -555,16 +564,16 @@ contains
    !> The scattering tables this program weights with, one line per entry:
    !! `neutron LABEL b` and `xray LABEL a1..a4 b1..b4 c`.
    subroutine print_elements()
-      real(rk) :: bval, coef(9)
-      logical :: ok
+      type(sqc_hip_species_info_t) :: sinfo
       integer :: k
-      do k = 1, n_neutron
-         call neutron_b(neutron_label(k), bval, ok)
-         write (output_unit, '(a,1x,a,1x,es24.16)') 'neutron', neutron_label(k), bval
+      do k = 0, species_table_size(0) - 1
+         call species_table_row(0, k, sinfo)
+         if (sinfo%has_neutron == 0) cycle
+         write (output_unit, '(a,1x,a,1x,es24.16)') 'neutron', info_label(sinfo), sinfo%neutron_b
       end do
-      do k = 1, n_xray
-         call xray_coefficients(xray_label(k), coef, ok)
-         write (output_unit, '(a,1x,a,9(1x,es24.16))') 'xray', xray_label(k), coef
+      do k = 0, species_table_size(1) - 1
+         call species_table_row(1, k, sinfo)
+         write (output_unit, '(a,1x,a,9(1x,es24.16))') 'xray', info_label(sinfo), sinfo%xray_coef
       end do
    end subroutine print_elements
 
@@ -598,8 +607,9 @@ contains
 
    subroutine parse_mapping(text)
       character(len=*), intent(in) :: text
-      integer :: p, q, colon, n, ios
-      character(len=:), allocatable :: item
+      integer :: p, q, colon, n, ios, ierr
+      character(len=:), allocatable :: item, message
+      type(sqc_hip_species_info_t) :: sinfo
       n = 1
       do p = 1, len_trim(text)
          if (text(p:p) == ',') n = n + 1
@@ -619,7 +629,11 @@ contains
                                                                '" (use TYPE:SPECIES, e.g. 1:Si)')
          read (item(:colon - 1), *, iostat=ios) map_type(a)
          if (ios /= 0 .or. map_type(a) < 1) call fail_usage('bad type id in mapping "'//item//'"')
-         map_label(a) = item(colon + 1:)
+         ! every label resolves to a species whatever the weights, and is kept
+         ! in its canonical spelling (src/sqc_weights.f90:49-93)
+         call resolve_species(item(colon + 1:), sinfo, ierr, message)
+         if (ierr /= 0) call fail_usage(message)
+         map_label(a) = info_label(sinfo)
       end do
    end subroutine parse_mapping
 

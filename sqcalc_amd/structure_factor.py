@@ -65,6 +65,9 @@ class HipStructureFactor:
         self.want_grid = bool(want_grid)
         self.qmin, self.qmax, self.nq = float(qmin), float(qmax), int(nq)
         S = len(self.species_type)
+        # every label of the mapping resolves to a species, whatever the
+        # weights (src/sqc_weights.f90:49-93); labels print canonically
+        self.mapping = {int(t): elements.canonical(lab) for t, lab in self.mapping.items()}
         if weight != "unit":
             missing = [int(t) for t in self.species_type if int(t) not in self.mapping]
             if missing:

@@ -35,20 +35,44 @@ def exe():
     return EXE
 
 
-def test_element_tables_match_python(exe):
-    from sqcalc_amd import elements
+def test_element_tables_match_reference_data(exe):
+    """`sqcalc_hip elements` lists what the Fortran host weights with: every
+    tabulated neutron length and IT92 row, against the reference's data
+    (tests/golden/elements/elements.json)."""
+    import json
+    from helpers import GOLDEN
+    with open(os.path.join(GOLDEN, "elements", "elements.json")) as fh:
+        d = json.load(fh)
     r = run(["elements"])
     assert r.returncode == 0, r.stderr
+    neutron = {e["symbol"]: e["neutron_b"] for e in d["elements"] if e["has_neutron"]}
+    xray = {sp["label"]: sp["coef"] for sp in d["species"]}
     seen_n, seen_x = set(), set()
     for line in r.stdout.splitlines():
         f = line.split()
         if f[0] == "neutron":
-            assert float(f[2]) == pytest.approx(elements.NEUTRON_B[f[1]], rel=1e-15)
+            assert float(f[2]) == pytest.approx(neutron[f[1]], rel=1e-15)
             seen_n.add(f[1])
         elif f[0] == "xray":
-            np.testing.assert_allclose([float(x) for x in f[2:]], elements.XRAY[f[1]], rtol=1e-15)
+            np.testing.assert_allclose([float(x) for x in f[2:]], xray[f[1]], rtol=1e-15)
             seen_x.add(f[1])
-    assert seen_n == set(elements.NEUTRON_B) and seen_x == set(elements.XRAY)
+    assert seen_n == set(neutron) and len(seen_n) == 86
+    assert seen_x == set(xray) and len(seen_x) == 213
+
+
+@pytest.mark.parametrize("weight,label,text", [
+    ("xray", "Es", "no X-ray form factor for Es; use --weight neutron or unit"),
+    ("neutron", "Fm", "no neutron scattering length for Fm; use --weight xray or unit"),
+])
+def test_species_without_data(exe, tmp_path, weight, label, text):
+    """The reference's validate_mapping (app/main.f90:515-551): a label that
+    resolves but has no data for the weights stops with code 15."""
+    d = load_golden("gas_unit_self")
+    p = tmp_path / "g.dump"
+    write_dump(p, d["frames"], np.ones_like(d["types"]), d["lo"], d["hi"], d["tilt"])
+    r = run(["static", "-i", str(p), "-m", "1:" + label, "-w", weight, "--quiet"])
+    assert r.returncode == 15, (r.returncode, r.stderr)
+    assert text in r.stderr
 
 
 @pytest.mark.parametrize("args,code,text", [
@@ -59,6 +83,11 @@ def test_element_tables_match_python(exe):
     (["static", "-i", "x.dump", "--device", "cpu"], 2, "GPU path only"),
     (["static", "-i", "x.dump", "-w", "photon"], 2, "unknown weight"),
     (["static", "-i", "x.dump", "-w", "neutron"], 2, "mapping"),
+    # the reference's label failures (src/sqc_elements.f90:241-291)
+    (["static", "-i", "x.dump", "-m", "1:Xq"], 2,
+     'unknown element symbol "Xq" in the element mapping'),
+    (["static", "-i", "x.dump", "-m", "1:Si,2:Fe7+"], 2,
+     'element "Fe" has no charge state "Fe7+"; available: Fe, Fe2+, Fe3+'),
     (["static", "-i", "x.dump", "--qmin", "5", "--qmax", "4"], 2, "larger than qmin"),
     (["static", "-i", "x.dump", "--precision", "half"], 2, "unknown precision"),
     (["static", "-i", "/nonexistent/x.dump"], 4, "cannot open"),
@@ -266,7 +295,7 @@ def test_xrd_fcc_nickel_lines(exe, hip_device, tmp_path):
     tab = np.array([[float(v) for v in l.split()] for l in lines[4:]])
     assert tab.shape == (200, 3)
     assert np.abs(tab[:, 1] - tab[:, 2]).max() <= 1e-9 * np.abs(tab[:, 1]).max()
-    c = elements.XRAY["Ni"]
+    c = elements.xray_coefficients("Ni")
 
     def f_ni(q):
         st = (q / (4.0 * np.pi)) ** 2
