@@ -26,11 +26,16 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_F64_PEAK_TFLOPS = 78.6
 COPY_BYTES = 1 << 30
+GRAD_EPS_F32 = 1e-7
+ERR_SHELLS = 40          # q < 0.8 at dq = 0.02: the oracle's direct sum finishes in seconds
 
 
 def main():
@@ -50,6 +55,10 @@ def main():
     ap.add_argument("--no-ingest", action="store_true",
                     help="skip the dump-reader and dump-to-S(q) measurement")
     ap.add_argument("--ingest-frames", type=int, default=64)
+    ap.add_argument("--no-dyn", action="store_true",
+                    help="skip the dyn amplitude leg (96k atoms, lattice modes to |q| = 2)")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the per-column error report against the FP64 oracle")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run an elementwise copy of 1 GiB (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -136,6 +145,9 @@ def main():
     # the whole frame against the same byte model
     frame_bytes = sum(nbytes[k] for k in ("spread_z", "dft_y", "dft_x_bin"))
     ref_bytes = _full_grid_bytes(info, wl.natoms, 8)
+    tables = {}
+    if not args.no_errors:
+        tables["fp32"] = _one_frame_table(sf, pos[0:1])
     sf.close()
 
     # ---- plain FP64 forward (sigma 2) ----
@@ -151,15 +163,17 @@ def main():
         sd.fetch()
         fp64 = nf / (time.perf_counter() - t0)
         fine64 = list(sd.info.fine)
+        if not args.no_errors:
+            tables["fp64"] = _one_frame_table(sd, pos[0:1])
         sd.close()
 
     # ---- forward + adjoint (S(q)-matching loss) ----
-    adj = adj_graph = None
+    adj = adj_graph = grad_err = None
     if not args.no_adjoint:
         from sqcalc_amd.autograd import sq_matching_loss
-        # gradient plans run at eps 1e-6 (the forward default 1e-5 leaves the
-        # FP32 gradient just above 1e-5 relative)
-        sa = make("single", partials=False, tune=True, eps=1e-6)
+        # FP32 gradient plans run at eps 1e-7 (ns 12): the kernel derivative's
+        # aliasing error sets the gradient's accuracy, 1e-5 relative there
+        sa = make("single", partials=False, tune=True, eps=GRAD_EPS_F32)
         tgt_sf = make("single", partials=False)
         tgt_sf.accumulate_device(pos[-1:])
         num, _, _, _ = tgt_sf.fetch()
@@ -201,6 +215,8 @@ def main():
             graph.replay()
         torch.cuda.synchronize()
         adj_graph = nf / (time.perf_counter() - t0)
+        if not args.no_errors:
+            grad_err = _gradient_error(sa, make, pos[0], target)
         sa.close()
 
     # ---- batched trajectory: 1000 frames through graph replays, autotuned tiles ----
@@ -212,6 +228,16 @@ def main():
     ingest = None
     if not args.no_ingest:
         ingest = _ingest(wl, host, args.ingest_frames, make, not args.no_cpu)
+
+    # ---- dyn: reciprocal-lattice amplitudes on the FP64 matrix cores ----
+    dyn = None
+    if not args.no_dyn:
+        dyn = _dyn_leg(wl, host, dev, not args.no_cpu)
+
+    # ---- per-column error of both precisions against the FP64 oracle ----
+    errors = None
+    if tables:
+        errors = _column_errors(wl, host[0], tables)
 
     # ---- CPU restatement of the reference FINUFFT path ----
     cpu = None
@@ -247,6 +273,10 @@ def main():
         "fp32_over_fp64": round(frames_per_s / fp64, 2) if fp64 else None,
         "fwd_adj_frames_per_s": round(adj, 2) if adj else None,
         "fwd_adj_graph_frames_per_s": round(adj_graph, 2) if adj else None,
+        "fwd_adj_eps": GRAD_EPS_F32,
+        "fwd_adj_grad_err_vs_fp64": grad_err,
+        "column_max_abs_err_vs_fp64_oracle": errors,
+        "dyn": dyn,
         "batched_trajectory": c3,
         "cpu_baseline": cpu,
         "ingest": ingest,
@@ -392,6 +422,127 @@ def _ingest(wl, host, nframes, make, with_reference):
         os.remove(os.path.join(tmp, name))
     os.rmdir(tmp)
     return out
+
+
+def _one_frame_table(sf, pos1):
+    """The plan's S(q) table of one frame (accumulators reset first)."""
+    sf.reset()
+    sf.accumulate_device(pos1)
+    return sf.table()
+
+
+def _column_errors(wl, frame, tables):
+    """Max |d| of every column (S, then the pair columns) of one frame's
+    table against the FP64 direct-sum oracle on the lowest ERR_SHELLS shells
+    (the shells are the same: both bin from q = 0 with dq = 0.02)."""
+    from oracle import oracle as O
+    dq = (wl.qmax - wl.qmin) / wl.nq
+    run = O.Run(wl.cell, wl.origin, wl.types, wl.mapping, weight=wl.weight, norm=wl.norm,
+                qmin=wl.qmin, qmax=wl.qmin + ERR_SHELLS * dq, nq=ERR_SHELLS)
+    run.add_frame(frame)
+    ref = run.table()
+    names = ["S"] + ["S(%s)" % l for l in ("Si-Si", "Si-O", "O-O")]
+    out = {"window": "%d shells, q < %.2f, %d modes" % (ERR_SHELLS, ERR_SHELLS * dq,
+                                                        int(run.md["hkl"].shape[0]))}
+    for key, tab in tables.items():
+        out[key] = {n: float("%.3e" % np.max(np.abs(tab[:ERR_SHELLS, c + 1] - ref[:, c + 1])))
+                    for c, n in enumerate(names)}
+    out["max_abs_ref"] = {n: float("%.4g" % np.max(np.abs(ref[:, c + 1])))
+                          for c, n in enumerate(names)}
+    return out
+
+
+def _gradient_error(sa, make, frame, target):
+    """The FP32 gradient plan against an FP64 one (eps 1e-13) on one frame,
+    max |g32 - g64| / max |g64|, two ways:
+      adjoint    the same upstream gradient gS = 2 (S64 - T) into both
+                 adjoints (what the hand-written adjoint adds);
+      loss       each plan's own S(q)-matching loss sum_s (S_s - T_s)^2,
+                 which also carries the FP32 forward's error of S into
+                 gS = 2 (S - T): large where |S - T| is small next to S
+                 (a target from another frame of the same lattice)."""
+    import torch
+
+    from sqcalc_amd.autograd import structure_factor
+    sd = make("double", partials=False, eps=1e-13, max_batch=1)
+    with torch.no_grad():
+        s64, _ = structure_factor(frame, sd)
+    gs = (2.0 * (s64 - target)).detach()
+    out = {}
+    for key, fixed in (("adjoint", True), ("loss", False)):
+        g = []
+        for plan in (sa, sd):
+            x = frame.clone().requires_grad_(True)
+            S, _ = structure_factor(x, plan)
+            (torch.sum(gs * S) if fixed else torch.sum((S - target) ** 2)).backward()
+            g.append(x.grad.cpu().numpy())
+        out[key] = float("%.3e" % (np.abs(g[0] - g[1]).max() / np.abs(g[1]).max()))
+    with torch.no_grad():
+        s32, _ = structure_factor(frame, sa)
+    out["max_abs_S_minus_T"] = float("%.3g" % (s64 - target).abs().max().item())
+    out["max_abs_S32_minus_S64"] = float("%.3g" % (s32 - s64).abs().max().item())
+    sd.close()
+    return out
+
+
+def _dyn_leg(wl, host, dev, with_cpu, qmax=2.0, fps=10, steps=5):
+    """The dyn amplitudes rho_a(n, t) of the 96,000-atom workload for every
+    reciprocal-lattice vector with 0 < |q| <= qmax (one of each +- pair, the
+    --qpoints grid sampling), FP64 on the matrix cores: frames/s with the
+    positions resident, the matrix kernel against the FP64 MFMA peak
+    (algorithmic flops 8 N nmodes per frame), the max deviation from the
+    oracle on a sample of modes, and the oracle (the reference's separable
+    phase algorithm restated, all host cores) on that sample, scaled to all
+    modes."""
+    import torch
+
+    from sqcalc_amd.dynamics import HipDynAmplitudes, grid_modes
+    b = 2.0 * np.pi * np.linalg.inv(wl.cell).T
+    hkl, _ = grid_modes(wl.cell, qmax)
+    dy = HipDynAmplitudes(b, wl.types, hkl)
+    info = dy.info
+    pos = torch.tensor(host[:fps], dtype=torch.float64, device=dev).contiguous()
+    out = dy.rho_device(pos)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dy.rho_device(pos, out=out)
+    torch.cuda.synchronize()
+    fr = fps * steps / (time.perf_counter() - t0)
+    dy.profile(True)
+    dy.rho_device(pos, out=out)
+    st = dy.stage_times()
+    dy.profile(False)
+    k_ms = st["dyn_mfma"][0] / st["dyn_mfma"][1]
+    alg = 8.0 * len(hkl) * wl.natoms
+    res = {"workload": "c2 geometry (96k atoms, 2 species), lattice modes 0 < |q| <= %.1f, "
+                       "one of each +- pair, FP64" % qmax,
+           "nmodes": int(len(hkl)), "frames_per_s": round(fr, 2),
+           "stage_ms": {k: round(v[0] / v[1], 4) for k, v in st.items()},
+           "roofline": {"bound": "mfma", "kernel": "k_dyn_rho",
+                        "achieved": round(alg / (k_ms * 1e-3) / 1e12, 2),
+                        "peak": MFMA_F64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(alg / (k_ms * 1e-3) / 1e12 / MFMA_F64_PEAK_TFLOPS, 4),
+                        "executed_TFLOPs": round(8.0 * info.macs_per_frame / (k_ms * 1e-3) / 1e12, 2),
+                        "avg_ms": round(k_ms, 4), "algorithmic_flops": alg}}
+    rho = out[0].cpu().numpy()
+    dy.close()
+    pick = np.sort(np.random.default_rng(7).choice(len(hkl), 4000, replace=False))
+    from oracle import oracle as O
+    sp = np.searchsorted(np.unique(wl.types), wl.types).astype(np.int32)
+    t1 = time.perf_counter()
+    ref = O.dyn_rho(host[0], sp, len(np.unique(wl.types)), b, hkl[pick])
+    dt = time.perf_counter() - t1
+    res["max_abs_err_vs_oracle"] = float("%.3e" % np.abs(rho[:, pick] - ref).max())
+    res["max_abs_rho"] = float("%.4g" % np.abs(ref).max())
+    if with_cpu:
+        per_frame = dt * len(hkl) / len(pick)
+        res["cpu_baseline"] = {"value": round(1.0 / per_frame, 4), "unit": "frames/s",
+                               "cores": O.lib().sqo_num_threads(), "kind": "port",
+                               "sample": "all %d atoms x %d of %d modes in %.2f s (separable "
+                                         "phase tables, src/sqc_dynamics.f90:826-851), scaled "
+                                         "to all modes" % (wl.natoms, len(pick), len(hkl), dt)}
+    return res
 
 
 def _cpu_baseline(wl, frame, nmodes):

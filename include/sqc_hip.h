@@ -218,6 +218,56 @@ const char *sqc_hip_last_error(void);
  * reference's opts ABI check (src/sqc_cufinufft.f90:277-290). */
 size_t sqc_hip_config_size(void);
 
+/* ---- dyn: density amplitudes on a set of Miller indices (FP64, MFMA) ----
+ *
+ * The amplitudes of the reference's dynamic method, per frame and species,
+ *   rho_a(n) = sum_{j in a} exp(i (n1 b1 + n2 b2 + n3 b3) . r_j + i base . r_j)
+ * for modes n = (n1, n2, n3): the reciprocal lattice (b = cell%b, no base)
+ * or a q line (b1 = dq u, base = s0 u).  Replaces the rho block of
+ * dyn_accumulate (src/sqc_dynamics.f90:826-873, with dyn_atom_phase :1094-1115
+ * and phase_tables / phase_factor of src/sqc_phase.f90:67-113); the time
+ * correlations of the amplitudes stay on the host.  Positions are Cartesian,
+ * unwrapped as the reference requires, (3, natoms) per frame. */
+typedef struct sqc_hip_dyn_config {
+  int32_t abi_version;      /* SQC_HIP_ABI_VERSION */
+  int32_t device;
+  int64_t natoms;
+  int32_t nspecies;
+  int32_t base_on;          /* 1: add base . r to every phase (q line) */
+  const int32_t *species_of; /* [natoms], 0-based */
+  double recip[9];          /* b1, b2, b3 in the columns (recip of dyn_adopt_modes) */
+  double base[3];
+  int64_t nmodes;
+  const int32_t *hkl;       /* [nmodes][3] Miller indices (mode_index) */
+} sqc_hip_dyn_config;
+
+typedef struct sqc_hip_dyn_plan sqc_hip_dyn_plan;
+
+typedef struct sqc_hip_dyn_info {
+  int64_t nmodes;
+  int64_t macs_per_frame;   /* complex multiply-adds the matrix cores run per frame */
+  int64_t computed_pairs;   /* (row, column) products per atom: live 16 x 16 tiles x 256 */
+  int64_t device_bytes;
+  int32_t nitems;           /* 16 x 128 blocks of the mode product */
+  int32_t nchunks;          /* atom chunks (per species) */
+  int32_t chunk_atoms;
+  int32_t column_axis;      /* Miller index that runs along the product's columns */
+} sqc_hip_dyn_info;
+
+int sqc_hip_dyn_plan_create(sqc_hip_dyn_plan **plan, const sqc_hip_dyn_config *cfg);
+int sqc_hip_dyn_plan_info(const sqc_hip_dyn_plan *plan, sqc_hip_dyn_info *info);
+/* rho_device[f][nspecies][nmodes] complex (re, im) FP64 for nframes frames of
+ * device positions, ordered on `stream` like sqc_hip_forward_device. */
+int sqc_hip_dyn_rho_device(sqc_hip_dyn_plan *plan, const double *pos_device, int64_t nframes,
+                           double *rho_device, void *stream);
+/* The same from and to host memory (synchronous). */
+int sqc_hip_dyn_rho(sqc_hip_dyn_plan *plan, const double *pos, int64_t nframes, double *rho);
+/* HIP-event times on the plan stream: 0 atom phases, 1 matrix product,
+ * 2 chunk sum and gather (as sqc_hip_profile / sqc_hip_stage_times). */
+int sqc_hip_dyn_profile(sqc_hip_dyn_plan *plan, int enable);
+int sqc_hip_dyn_stage_times(sqc_hip_dyn_plan *plan, double *ms, int64_t *launches, int nstage);
+int sqc_hip_dyn_destroy(sqc_hip_dyn_plan *plan);
+
 /* ---- Species labels and their scattering data (host side, no device) ----
  *
  * The -m TYPE:LABEL mapping resolves each label to an element and, for the

@@ -94,6 +94,9 @@ def lib():
     L.sqo_nufft_bin.argtypes = [ctypes.c_int, _d, _i64, ctypes.c_int64, _i32, _d, _i32,
                                 ctypes.c_int, _d, _d, _d, ctypes.c_int, _d, _d, _d, _d]
     L.sqo_num_threads.restype = ctypes.c_int
+    L.sqo_dyn_rho.restype = ctypes.c_int
+    L.sqo_dyn_rho.argtypes = [ctypes.c_int64, _d, _i32, ctypes.c_int, _d, _d, ctypes.c_int,
+                              ctypes.c_int64, _i32, ctypes.c_int, _d]
     _LIB = L
     return L
 
@@ -440,3 +443,25 @@ def nufft_frame(run, pos, eps=1e-9, upsampfac=1.25, nthreads=None, fields=None):
                     _p(_f64(run.xcoef), _d), _p(num, _d),
                     _p(part, _d) if run.partials else None)
     return num, (part if run.partials else None), {"n": n, "ns": ns, "beta": beta}
+
+
+# --------------------------------------------------------------------------
+# dyn: density amplitudes on a list of Miller indices
+# --------------------------------------------------------------------------
+
+def dyn_rho(pos, species_of, nspecies, recip, hkl, base=None, method=1):
+    """rho[a][m] complex of exp(i (n_m . b) . r_j [+ i base . r_j]) summed over
+    the atoms of species a: method 1 the reference's separable tables, 0 its
+    direct cos/sin branch (sqo_dyn_rho, src/sqc_dynamics.f90:826-873)."""
+    pos = _f64(np.asarray(pos).reshape(-1, 3))
+    sp = np.ascontiguousarray(species_of, dtype=np.int32)
+    hkl = np.ascontiguousarray(hkl, dtype=np.int32).reshape(-1, 3)
+    recip = _f64(np.asarray(recip).reshape(3, 3).ravel(order="F"))
+    bv = _f64(np.zeros(3) if base is None else base)
+    out = np.zeros((nspecies, hkl.shape[0], 2))
+    rc = lib().sqo_dyn_rho(len(sp), _p(pos, _d), _p(sp, _i32), nspecies, _p(recip, _d),
+                           _p(bv, _d), 0 if base is None else 1, hkl.shape[0], _p(hkl, _i32),
+                           method, _p(out, _d))
+    if rc:
+        raise MemoryError("oracle dyn_rho")
+    return out[..., 0] + 1j * out[..., 1]

@@ -443,3 +443,142 @@ int sqo_num_threads(void) {
   return 1;
 #endif
 }
+
+/* ---------------------------------------------------------------------- */
+/* dyn: density amplitudes on a list of Miller indices                      */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+  double re, im;
+} sqo_cplx;
+
+static sqo_cplx cmul_(sqo_cplx a, sqo_cplx b) {
+  sqo_cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return r;
+}
+
+/* One atom's three factor tables, t[j][n + nmax[j]] = base_j exp(i n phi_j)
+ * for n = -nmax_j .. nmax_j (phase_tables, src/sqc_phase.f90:67-99): the
+ * positive half by the recurrence prod *= u, the negative half by its own
+ * recurrence with conj(u) when the axis has a base, else as the conjugate of
+ * the positive half.  width = 2 max(nmax) + 1 entries per axis. */
+static void dyn_tables(sqo_cplx *t, int width, const double *phi, const int *nmax,
+                       const sqo_cplx *base, int base_on) {
+  for (int j = 0; j < 3; ++j) {
+    sqo_cplx *tj = t + (size_t)j * width;
+    const int off = nmax[j] > 0 ? nmax[j] : 0;
+    const sqo_cplx u = {cos(phi[j]), sin(phi[j])};
+    const sqo_cplx c0 = base_on ? base[j] : (sqo_cplx){1.0, 0.0};
+    sqo_cplx prod = c0;
+    tj[off] = c0;
+    for (int n = 1; n <= off; ++n) {
+      prod = cmul_(prod, u);
+      tj[off + n] = prod;
+    }
+    if (base_on) {
+      const sqo_cplx uc = {u.re, -u.im};
+      prod = c0;
+      for (int n = 1; n <= off; ++n) {
+        prod = cmul_(prod, uc);
+        tj[off - n] = prod;
+      }
+    } else {
+      for (int n = 1; n <= off; ++n) {
+        tj[off - n].re = tj[off + n].re;
+        tj[off - n].im = -tj[off + n].im;
+      }
+    }
+  }
+}
+
+/* rho[a][m] (re, im) = sum over the atoms of species a of exp(i q_m . r_j),
+ * q_m = hkl_m1 b1 + hkl_m2 b2 + hkl_m3 b3 (b in the columns of recip), plus
+ * base . r when base_on (a q line's start).  Restates the rho block of
+ * dyn_accumulate (src/sqc_dynamics.f90:826-873): method 1 the separable
+ * branch (per atom: phi_k = recip(1,k) r1 + recip(2,k) r2 + recip(3,k) r3,
+ * dyn_atom_phase :1094-1115, the tables above, and rho += t1 t2 t3 in that
+ * order, phase_factor src/sqc_phase.f90:103-113); method 0 the direct
+ * branch (:853-873, one cos/sin of q . r per mode and atom, with
+ * q = n1 b1 + n2 b2 + n3 b3 and the base added).  Atoms are split over the
+ * threads in order and the per-thread sums added in thread order, as the
+ * reference reduces rho_thread. */
+int sqo_dyn_rho(int64_t natoms, const double *pos, const int32_t *species_of, int nspecies,
+                const double *recip, const double *base_vec, int base_on, int64_t nmodes,
+                const int32_t *hkl, int method, double *rho) {
+  int nmax[3] = {0, 0, 0};
+  for (int64_t m = 0; m < nmodes; ++m)
+    for (int j = 0; j < 3; ++j) {
+      const int v = hkl[3 * m + j] < 0 ? -hkl[3 * m + j] : hkl[3 * m + j];
+      if (v > nmax[j]) nmax[j] = v;
+    }
+  int mx = nmax[0];
+  if (nmax[1] > mx) mx = nmax[1];
+  if (nmax[2] > mx) mx = nmax[2];
+  const int width = 2 * mx + 1;
+  const size_t nr = (size_t)nspecies * nmodes;
+  int nthr = 1;
+#ifdef _OPENMP
+  nthr = omp_get_max_threads();
+#endif
+  double *acc = (double *)calloc((size_t)nthr * nr * 2, sizeof(double));
+  if (!acc) return 1;
+  int fail = 0;
+#pragma omp parallel num_threads(nthr)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double *mine = acc + (size_t)tid * nr * 2;
+    sqo_cplx *t = (sqo_cplx *)malloc(sizeof(sqo_cplx) * 3 * width);
+    if (!t) {
+#pragma omp atomic write
+      fail = 1;
+    }
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < natoms; ++i) {
+      if (!t) continue;
+      const double *r = pos + 3 * i;
+      const int a = species_of[i];
+      double phi[3];
+      for (int k = 0; k < 3; ++k)
+        phi[k] = recip[0 + 3 * k] * r[0] + recip[1 + 3 * k] * r[1] + recip[2 + 3 * k] * r[2];
+      double d = 0.0;
+      if (base_on) d = base_vec[0] * r[0] + base_vec[1] * r[1] + base_vec[2] * r[2];
+      if (method == 1) {
+        sqo_cplx base[3] = {{cos(d), sin(d)}, {1.0, 0.0}, {1.0, 0.0}};
+        dyn_tables(t, width, phi, nmax, base, base_on);
+        for (int64_t m = 0; m < nmodes; ++m) {
+          const int32_t *n = hkl + 3 * m;
+          const sqo_cplx v = cmul_(cmul_(t[n[0] + nmax[0]], t[width + n[1] + nmax[1]]),
+                                   t[2 * width + n[2] + nmax[2]]);
+          double *o = mine + 2 * ((size_t)a * nmodes + m);
+          o[0] += v.re;
+          o[1] += v.im;
+        }
+      } else {
+        for (int64_t m = 0; m < nmodes; ++m) {
+          const int32_t *n = hkl + 3 * m;
+          double q[3];
+          for (int c = 0; c < 3; ++c)
+            q[c] = recip[c] * n[0] + recip[c + 3] * n[1] + recip[c + 6] * n[2];
+          double ph = q[0] * r[0] + q[1] * r[1] + q[2] * r[2];
+          if (base_on) ph += d;
+          double *o = mine + 2 * ((size_t)a * nmodes + m);
+          o[0] += cos(ph);
+          o[1] += sin(ph);
+        }
+      }
+    }
+    free(t);
+  }
+  if (!fail) {
+    for (size_t k = 0; k < 2 * nr; ++k) {
+      double s = 0.0;
+      for (int th = 0; th < nthr; ++th) s += acc[(size_t)th * nr * 2 + k];
+      rho[k] = s;
+    }
+  }
+  free(acc);
+  return fail;
+}

@@ -47,6 +47,23 @@ class Info(_c.Structure):
     ]
 
 
+class DynConfig(_c.Structure):
+    _fields_ = [
+        ("abi_version", _c.c_int32), ("device", _c.c_int32), ("natoms", _c.c_int64),
+        ("nspecies", _c.c_int32), ("base_on", _c.c_int32),
+        ("species_of", _c.POINTER(_c.c_int32)), ("recip", _c.c_double * 9),
+        ("base", _c.c_double * 3), ("nmodes", _c.c_int64), ("hkl", _c.POINTER(_c.c_int32)),
+    ]
+
+
+class DynInfo(_c.Structure):
+    _fields_ = [
+        ("nmodes", _c.c_int64), ("macs_per_frame", _c.c_int64), ("computed_pairs", _c.c_int64),
+        ("device_bytes", _c.c_int64), ("nitems", _c.c_int32), ("nchunks", _c.c_int32),
+        ("chunk_atoms", _c.c_int32), ("column_axis", _c.c_int32),
+    ]
+
+
 class SpeciesInfo(_c.Structure):
     _fields_ = [
         ("label", _c.c_char * 16), ("symbol", _c.c_char * 4), ("z", _c.c_int32),
@@ -88,6 +105,13 @@ SIGNATURES = {
     "sqc_hip_error_string": (_c.c_char_p, [_c.c_int]),
     "sqc_hip_last_error": (_c.c_char_p, []),
     "sqc_hip_config_size": (_c.c_size_t, []),
+    "sqc_hip_dyn_plan_create": (_c.c_int, [_c.POINTER(_vp), _c.POINTER(DynConfig)]),
+    "sqc_hip_dyn_plan_info": (_c.c_int, [_vp, _c.POINTER(DynInfo)]),
+    "sqc_hip_dyn_rho_device": (_c.c_int, [_vp, _vp, _c.c_int64, _vp, _vp]),
+    "sqc_hip_dyn_rho": (_c.c_int, [_vp, _vp, _c.c_int64, _vp]),
+    "sqc_hip_dyn_profile": (_c.c_int, [_vp, _c.c_int]),
+    "sqc_hip_dyn_stage_times": (_c.c_int, [_vp, _vp, _vp, _c.c_int]),
+    "sqc_hip_dyn_destroy": (_c.c_int, [_vp]),
     "sqc_hip_species_lookup": (_c.c_int, [_c.c_char_p, _c.POINTER(SpeciesInfo)]),
     "sqc_hip_species_table": (_c.c_int, [_c.c_int32, _c.c_int32, _c.POINTER(SpeciesInfo),
                                          _c.POINTER(_c.c_int32)]),

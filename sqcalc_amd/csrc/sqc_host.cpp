@@ -626,7 +626,10 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if (!(sigma > 1.0)) return fail(SQC_HIP_ERR_BAD_ARG, "upsampfac must exceed 1");
   p.sigma = sigma;
   int ret = SQC_HIP_OK;
-  const double emin = p.prec == 1 ? 1.2e-7 : 2.3e-16;
+  // FP32 forward sums stop gaining below about 1e-7, but the adjoint's
+  // kernel-derivative aliasing keeps falling with eps (down to the ns = 16
+  // cap), so FP32 plans take eps down to 1e-10 as well
+  const double emin = p.prec == 1 ? 1e-10 : 2.3e-16;
   if (eps < emin) {
     eps = emin;
     ret = SQC_HIP_WARN_EPS_TOO_SMALL;
@@ -1039,6 +1042,12 @@ int run_frames(sqc::Plan &p, const double *pos_dev, int64_t nframes) {
 }
 
 }  // namespace
+
+namespace sqc {
+// The thread's error message for sqc_hip_last_error, set by the other
+// translation units of the library.
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace sqc
 
 extern "C" {
 

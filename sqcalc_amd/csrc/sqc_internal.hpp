@@ -171,6 +171,8 @@ int launch_forward_y(Plan &p, const void *W1, void *W2, hipStream_t st);
 int launch_forward_x(Plan &p, const void *W2, hipStream_t st, int mode, double *frame_out,
                      void *rho_out);
 int launch_finalize(Plan &p, double *frame_out, int accumulate, hipStream_t st);
+// Record msg as the thread's sqc_hip_last_error() text; returns code.
+int set_error(int code, const std::string &msg);
 int kernel_lds_limits(Plan &p);
 // Static LDS (bytes) of the spread and gather kernels (their staging block)
 // and of the x passes (their line-group table): a launch's dynamic LDS must
