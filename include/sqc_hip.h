@@ -50,7 +50,8 @@ enum {
   SQC_HIP_ERR_HIP = 32,             /* a HIP runtime call failed */
   SQC_HIP_ERR_NO_MODES = 33,        /* no reciprocal lattice point in [qmin, qmax] */
   SQC_HIP_ERR_UNKNOWN_ELEMENT = 34, /* species label names no element */
-  SQC_HIP_ERR_NO_CHARGE_STATE = 35  /* element known, charge or valence state not tabulated */
+  SQC_HIP_ERR_NO_CHARGE_STATE = 35, /* element known, charge or valence state not tabulated */
+  SQC_HIP_ERR_POSITIONS = 36        /* Debye: an atom outside a non-periodic box, or adrift */
 };
 
 enum { SQC_HIP_F64 = 0, SQC_HIP_F32 = 1 };
@@ -267,6 +268,71 @@ int sqc_hip_dyn_rho(sqc_hip_dyn_plan *plan, const double *pos, int64_t nframes, 
 int sqc_hip_dyn_profile(sqc_hip_dyn_plan *plan, int enable);
 int sqc_hip_dyn_stage_times(sqc_hip_dyn_plan *plan, double *ms, int64_t *launches, int nstage);
 int sqc_hip_dyn_destroy(sqc_hip_dyn_plan *plan);
+
+/* ---- Debye: ordered pair-distance histograms per type pair (FP64) ----
+ *
+ * The pair counts of the reference's Debye method, histogram(a, b, k) = the
+ * number of ordered pairs (i, j), i of type a, j of type b, whose distance
+ * falls in bin k = int(|d| / dr) + 1 <= nbins = max(1, ceiling(rmax / dr)),
+ * |d| <= rmax, summed over the frames.  Replaces debye_accumulate
+ * (src/sqc_debye.f90:179-235) with the displacement of its cell list
+ * (nl_begin_frame / nl_build, src/sqc_neighbour_list.f90:190-330): every pair
+ * lands in the bin the reference puts it in.  Set-up follows debye_setup
+ * (src/sqc_debye.f90:95-177): the same checks and messages, rmax below the
+ * smallest periodic width.  The cell and origin are those of the first frame
+ * (the reference's list keeps them); positions are Cartesian, (3, natoms)
+ * per frame, in any wrapping. */
+typedef struct sqc_hip_debye_config {
+  int32_t abi_version;      /* SQC_HIP_ABI_VERSION */
+  int32_t device;
+  int64_t natoms;
+  int32_t ntypes;           /* type ids run 1..ntypes */
+  int32_t pbc[3];           /* periodicity of each cell vector */
+  const int32_t *type_id;   /* [natoms], 1-based as in the dump */
+  double cell[9];           /* lattice vectors a1, a2, a3 in the columns */
+  double origin[3];
+  double rmax;              /* pair cutoff (> 0; the caller resolves the default) */
+  double dr;                /* bin width */
+  double skin;              /* Verlet skin of the reference's list: sets its cells */
+} sqc_hip_debye_config;
+
+typedef struct sqc_hip_debye_plan sqc_hip_debye_plan;
+
+typedef struct sqc_hip_debye_info {
+  int32_t nbins;
+  int32_t ntypes;
+  int32_t div[3];           /* the reference list's cells per cell vector */
+  int32_t hist_in_lds;      /* 1: each workgroup counts into an LDS histogram */
+  int32_t workgroups;
+  int32_t reserved;
+  int64_t pairs_per_frame;  /* unordered atom pairs evaluated per frame */
+  int64_t device_bytes;
+  int64_t nframes;          /* frames counted since create / reset */
+} sqc_hip_debye_info;
+
+int sqc_hip_debye_plan_create(sqc_hip_debye_plan **plan, const sqc_hip_debye_config *cfg);
+int sqc_hip_debye_plan_info(const sqc_hip_debye_plan *plan, sqc_hip_debye_info *info);
+/* Count nframes frames of device positions ([f][natoms][3]) on the plan's
+ * stream; positions are read before the call's work ends (synchronise
+ * with sqc_hip_debye_fetch before reusing them). */
+int sqc_hip_debye_accumulate_device(sqc_hip_debye_plan *plan, const double *pos_device,
+                                    int64_t nframes);
+/* The same from host memory (synchronous). */
+int sqc_hip_debye_accumulate(sqc_hip_debye_plan *plan, const double *pos, int64_t nframes);
+/* histogram[a][b][k] (ntypes x ntypes x nbins, C order, 0-based) of ordered
+ * pair counts summed over the frames, as doubles like the reference's
+ * histogram before its division by nhist; *nframes the frames counted.
+ * Either pointer may be NULL. */
+int sqc_hip_debye_fetch(sqc_hip_debye_plan *plan, double *histogram, int64_t *nframes);
+/* SQC_HIP_ERR_POSITIONS from fetch (and from the synchronous accumulate):
+ * some frame had an atom outside the box along a non-periodic axis (the
+ * reference's "an atom lies outside a non-periodic box", which it reports
+ * on the frames that rebuild its list) or 2^20 cells away from its first
+ * image; such an atom's pairs are not counted along that axis. */
+/* Clear the counts, the frame count and the position flags (the list's
+ * unwrapped positions carry on, as the reference's do). */
+int sqc_hip_debye_reset(sqc_hip_debye_plan *plan);
+int sqc_hip_debye_destroy(sqc_hip_debye_plan *plan);
 
 /* ---- Species labels and their scattering data (host side, no device) ----
  *

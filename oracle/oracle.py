@@ -97,6 +97,13 @@ def lib():
     L.sqo_dyn_rho.restype = ctypes.c_int
     L.sqo_dyn_rho.argtypes = [ctypes.c_int64, _d, _i32, ctypes.c_int, _d, _d, ctypes.c_int,
                               ctypes.c_int64, _i32, ctypes.c_int, _d]
+    _u64 = ctypes.POINTER(ctypes.c_uint64)
+    L.sqo_cell_widths.argtypes = [_d, _d]
+    L.sqo_debye_cells.argtypes = [_d, ctypes.c_double, ctypes.c_double, _i32, _d]
+    L.sqo_debye_positions.argtypes = [ctypes.c_int64, _d, _d, _d, ctypes.c_int, _d, _d]
+    L.sqo_debye_hist.restype = ctypes.c_int
+    L.sqo_debye_hist.argtypes = [ctypes.c_int64, ctypes.c_int, _i32, _d, _i32, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, _d, ctypes.c_int, _u64]
     _LIB = L
     return L
 
@@ -465,3 +472,145 @@ def dyn_rho(pos, species_of, nspecies, recip, hkl, base=None, method=1):
     if rc:
         raise MemoryError("oracle dyn_rho")
     return out[..., 0] + 1j * out[..., 1]
+
+
+# --------------------------------------------------------------------------
+# Debye pair histograms and the S(q) / g(r) evaluated from them
+# --------------------------------------------------------------------------
+
+def cell_widths(cell):
+    a = _f64(np.asarray(cell).reshape(3, 3).ravel(order="F"))
+    w = np.zeros(3)
+    lib().sqo_cell_widths(_p(a, _d), _p(w, _d))
+    return w
+
+
+def debye_cells(cell, rmax, skin):
+    """(div[3], bins (3, 3)) of the reference's cell list (nl_configure)."""
+    a = _f64(np.asarray(cell).reshape(3, 3).ravel(order="F"))
+    div = np.zeros(3, dtype=np.int32)
+    bins = np.zeros(9)
+    lib().sqo_debye_cells(_p(a, _d), rmax, skin, _p(div, _i32), _p(bins, _d))
+    return div, bins.reshape(3, 3, order="F")
+
+
+class DebyeRun:
+    """Ordered pair counts of a trajectory as the reference's Debye method
+    forms them (sqo_debye_positions + sqo_debye_hist per frame):
+    hist[a][b][k], 0-based types, uint64, summed over the frames."""
+
+    def __init__(self, cell, origin, types, ntypes, pbc=(1, 1, 1), rmax=None, dr=0.01, skin=1.0):
+        self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+        self.a = _f64(self.cell.ravel(order="F"))
+        self.origin = _f64(origin)
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        self.ntypes = int(ntypes)
+        self.pbc = np.ascontiguousarray([1 if p else 0 for p in pbc], dtype=np.int32)
+        self.rmax, self.dr, self.skin = float(rmax), float(dr), float(skin)
+        self.nbins = max(1, int(math.ceil(self.rmax / self.dr)))
+        n = len(self.types)
+        self.pos = np.zeros((n, 3))
+        self.posw = np.zeros((n, 3))
+        self.nframes = 0
+        self.hist = np.zeros((self.ntypes, self.ntypes, self.nbins), dtype=np.uint64)
+
+    def add_frame(self, r, method=1):
+        r = _f64(np.asarray(r).reshape(-1, 3))
+        L = lib()
+        L.sqo_debye_positions(len(self.types), _p(self.a, _d), _p(self.origin, _d), _p(r, _d),
+                              1 if self.nframes == 0 else 0, _p(self.pos, _d), _p(self.posw, _d))
+        rc = L.sqo_debye_hist(len(self.types), self.ntypes, _p(self.types, _i32), _p(self.a, _d),
+                              _p(self.pbc, _i32), self.rmax, self.dr, self.skin, _p(self.pos, _d),
+                              method, self.hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        if rc:
+            raise MemoryError("oracle debye_hist")
+        self.nframes += 1
+
+
+def debye_sq(hist, nhist, counts, natoms, volume, pbc, rmax, dr, qmin, qmax, nq, kind, amp,
+             xcoef, norm="mean", partials=True, correct_cutoff=True):
+    """(num[nq], den[nq], partial_num[T][T][nq]) of debye_prepare_output
+    (src/sqc_debye.f90:236-345) for nhist frames of ordered pair counts
+    hist[a][b][k] of types with amplitudes amp / xcoef: the pair sum over
+    bin centres r_k = (k - 1/2) dr, the r = 0 self term, debyer's cut-off
+    correction; the table is num / (nframes den) as sf_shell_value."""
+    hist = np.asarray(hist, dtype=np.float64)
+    T, _, nbins = hist.shape
+    counts = np.asarray(counts, dtype=np.int64)
+    dq = (qmax - qmin) / nq
+    four_pi = 4.0 * math.acos(-1.0)
+    r_bin = (np.arange(1, nbins + 1) - 0.5) * dr
+    num, den = np.zeros(nq), np.zeros(nq)
+    part = np.zeros((T, T, nq)) if partials else None
+    for s in range(nq):
+        q = qmin + (s + 1 - 0.5) * dq
+        w = np.array([float(amplitude(kind, amp, xcoef, t, q)) for t in range(T)])
+        sinc = np.sin(q * r_bin) / (q * r_bin)
+        pair_sum = 0.0
+        for a in range(T):
+            if counts[a] == 0:
+                continue
+            for b in range(T):
+                if counts[b] == 0:
+                    continue
+                h = hist[a, b]
+                keep = h > 0.0
+                pair_sum += float(np.sum(w[a] * w[b] * (h[keep] / nhist) * sinc[keep]))
+        live = counts > 0
+        self_sum = float(np.sum(counts[live] * w[live] * w[live]))
+        weight_sum = float(np.sum(counts * w))
+        if norm == "n":
+            den[s] = float(natoms)
+        elif norm == "mean":
+            sw = float(np.sum(counts[live] * w[live]))
+            den[s] = sw * sw / natoms
+        else:
+            den[s] = self_sum
+        cut = correct_cutoff and any(pbc)
+        if partials:
+            pcorr = 0.0
+            if cut:
+                qc = q * rmax
+                pcorr = four_pi * (natoms / volume) / (q * q) * (rmax * math.cos(qc) - math.sin(qc) / q)
+            for a in range(T):
+                if counts[a] == 0:
+                    continue
+                for b in range(a, T):
+                    if counts[b] == 0:
+                        continue
+                    h = hist[a, b]
+                    keep = h > 0.0
+                    ps = float(np.sum((h[keep] / nhist) * sinc[keep]))
+                    if a == b:
+                        ps += float(counts[a])
+                    xa, xb = counts[a] / natoms, counts[b] / natoms
+                    ps += xa * xb * pcorr * natoms
+                    part[a, b, s] = ps * nhist
+        corr = 0.0
+        if cut:
+            rho0 = natoms / volume
+            avg = weight_sum / natoms
+            qc = q * rmax
+            corr = avg ** 2 * four_pi * rho0 / (q * q) * (rmax * math.cos(qc) - math.sin(qc) / q)
+        num[s] = (pair_sum + self_sum + corr * natoms) * nhist if den[s] > 0.0 else 0.0
+    return num, den, part
+
+
+def debye_g(hist, nhist, counts, volume, dr, exact_shell=False):
+    """(r[nbins], g[T][T][nbins]) of debye_fill_g_partial (src/sqc_debye.f90:422-452)."""
+    hist = np.asarray(hist, dtype=np.float64)
+    T, _, nbins = hist.shape
+    r = (np.arange(1, nbins + 1) - 0.5) * dr
+    if exact_shell:
+        rlo = np.maximum(0.0, r - 0.5 * dr)
+        shell = 4.0 * math.acos(-1.0) / 3.0 * ((r + 0.5 * dr) ** 3 - rlo ** 3)
+    else:
+        shell = 4.0 * math.acos(-1.0) * r ** 2 * dr
+    g = np.zeros((T, T, nbins))
+    for a in range(T):
+        for b in range(T):
+            if counts[a] == 0 or counts[b] == 0:
+                continue
+            rho_b = counts[b] / volume
+            g[a, b] = (hist[a, b] / nhist) / (counts[a] * shell * rho_b)
+    return r, g

@@ -582,3 +582,158 @@ int sqo_dyn_rho(int64_t natoms, const double *pos, const int32_t *species_of, in
   free(acc);
   return fail;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Debye: ordered pair-distance histograms per type pair                    */
+/* ---------------------------------------------------------------------- */
+
+/* Perpendicular widths volume / |a_j x a_k|, src/sqc_cell.f90:100-117. */
+void sqo_cell_widths(const double *a, double *w) {
+  const double vol = fabs(det3(a));
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (j + 1) % 3;
+    const double *u = a + 3 * j, *v = a + 3 * k;
+    const double n0 = u[1] * v[2] - u[2] * v[1];
+    const double n1 = u[2] * v[0] - u[0] * v[2];
+    const double n2 = u[0] * v[1] - u[1] * v[0];
+    w[i] = vol / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  }
+}
+
+/* The cell list's decomposition, nl_configure (src/sqc_neighbour_list.f90:
+ * 109-123): div_i = max(1, int(width_i / (rmax + max(skin, 0)))) and
+ * bins(:, i) = a_i / div_i (column-major). */
+void sqo_debye_cells(const double *a, double rmax, double skin, int32_t *div, double *bins) {
+  double w[3];
+  sqo_cell_widths(a, w);
+  const double reach = rmax + (skin > 0.0 ? skin : 0.0);
+  for (int i = 0; i < 3; ++i) {
+    const int d = (int)(w[i] / reach);
+    div[i] = d > 1 ? d : 1;
+    for (int c = 0; c < 3; ++c) M(bins, c, i) = M(a, c, i) / (double)div[i];
+  }
+}
+
+/* The list's positions of one frame, nl_begin_frame (src/sqc_neighbour_list.f90
+ * :211-229).  s = wrap(A^-1 (r - o)), r_new = A s; the first frame takes
+ * pos = posw = r_new, later frames pos += A (ds - anint(ds)) with
+ * ds = A^-1 (r_new - posw) and posw = r_new. */
+void sqo_debye_positions(int64_t natoms, const double *a, const double *origin, const double *r,
+                         int first, double *pos, double *posw) {
+  double inv[9];
+  sqo_inverse3(a, inv);
+  for (int64_t i = 0; i < natoms; ++i) {
+    double s[3], rn[3];
+    frac_wrap(inv, origin, r + 3 * i, s);
+    for (int c = 0; c < 3; ++c) rn[c] = M(a, c, 0) * s[0] + M(a, c, 1) * s[1] + M(a, c, 2) * s[2];
+    double *p = pos + 3 * i, *pw = posw + 3 * i;
+    if (first) {
+      for (int c = 0; c < 3; ++c) p[c] = pw[c] = rn[c];
+      continue;
+    }
+    double dr[3], ds[3];
+    for (int c = 0; c < 3; ++c) dr[c] = rn[c] - pw[c];
+    for (int c = 0; c < 3; ++c) ds[c] = M(inv, c, 0) * dr[0] + M(inv, c, 1) * dr[1] + M(inv, c, 2) * dr[2];
+    for (int c = 0; c < 3; ++c) ds[c] = ds[c] - round(ds[c]);
+    for (int c = 0; c < 3; ++c) dr[c] = M(a, c, 0) * ds[0] + M(a, c, 1) * ds[1] + M(a, c, 2) * ds[2];
+    for (int c = 0; c < 3; ++c) {
+      p[c] = p[c] + dr[c];
+      pw[c] = rn[c];
+    }
+  }
+}
+
+/* Ordered pair counts of one frame of list positions, the pair loop of
+ * debye_accumulate (src/sqc_debye.f90:208-226) over every (i, j, image m)
+ * the cell list holds: d = (pos_j - pos_i) + bins . (div * m), counted when
+ * d . d <= rmax^2 in bin int(sqrt(d . d) / dr) + 1 <= nbins, i == j only
+ * with m != 0.  The list keeps exactly the images within reach of each
+ * pair (nl_build :263-330), and |d| <= rmax bounds the fractional
+ * displacement, |s_k + m_k| <= rmax / width_k, so the images are taken from
+ * that bound: method 1 per pair from s = A^-1 pos_j - A^-1 pos_i, method 0
+ * from a box of +-(ceil(rmax / width_k) + 1) around the minimum image (brute
+ * force).  hist[a][b][k] (0-based, ntypes x ntypes x nbins) gains the counts;
+ * type_id is 1-based. */
+int sqo_debye_hist(int64_t natoms, int ntypes, const int32_t *type_id, const double *a,
+                   const int32_t *pbc, double rmax, double dr, double skin, const double *pos,
+                   int method, uint64_t *hist) {
+  double inv[9], w[3], bins[9];
+  int32_t div[3];
+  sqo_inverse3(a, inv);
+  sqo_cell_widths(a, w);
+  sqo_debye_cells(a, rmax, skin, div, bins);
+  const double inv_dr = 1.0 / dr;
+  const double rr = rmax * rmax;
+  const int nbins = (int)ceil(rmax / dr) > 1 ? (int)ceil(rmax / dr) : 1;
+  const size_t nh = (size_t)ntypes * ntypes * nbins;
+  double *frac = (double *)malloc(sizeof(double) * 3 * (size_t)natoms);
+  const int nthr = sqo_num_threads();
+  uint64_t *acc = (uint64_t *)calloc(nh * (size_t)nthr, sizeof(uint64_t));
+  if (!frac || !acc) {
+    free(frac);
+    free(acc);
+    return 1;
+  }
+  for (int64_t i = 0; i < natoms; ++i)
+    for (int c = 0; c < 3; ++c)
+      frac[3 * i + c] = M(inv, c, 0) * pos[3 * i] + M(inv, c, 1) * pos[3 * i + 1] +
+                        M(inv, c, 2) * pos[3 * i + 2];
+  double reach[3];
+  int wide[3];
+  for (int k = 0; k < 3; ++k) {
+    reach[k] = rmax / w[k] + 1e-9;
+    wide[k] = (int)ceil(rmax / w[k]) + 1;
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < natoms; ++i) {
+#ifdef _OPENMP
+    uint64_t *mine = acc + nh * (size_t)omp_get_thread_num();
+#else
+    uint64_t *mine = acc;
+#endif
+    const int ta = type_id[i] - 1;
+    for (int64_t j = 0; j < natoms; ++j) {
+      int lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        const double s = frac[3 * j + k] - frac[3 * i + k];
+        if (!pbc[k]) {
+          lo[k] = hi[k] = 0;
+        } else if (method == 1) {
+          lo[k] = (int)ceil(-s - reach[k]);
+          hi[k] = (int)floor(-s + reach[k]);
+        } else {
+          const int c0 = -(int)round(s);
+          lo[k] = c0 - wide[k];
+          hi[k] = c0 + wide[k];
+        }
+      }
+      const double e0 = pos[3 * j] - pos[3 * i], e1 = pos[3 * j + 1] - pos[3 * i + 1],
+                   e2 = pos[3 * j + 2] - pos[3 * i + 2];
+      const int tb = type_id[j] - 1;
+      for (int m2 = lo[2]; m2 <= hi[2]; ++m2)
+        for (int m1 = lo[1]; m1 <= hi[1]; ++m1)
+          for (int m0 = lo[0]; m0 <= hi[0]; ++m0) {
+            if (i == j && m0 == 0 && m1 == 0 && m2 == 0) continue;
+            const double c0 = (double)(div[0] * m0), c1 = (double)(div[1] * m1),
+                         c2 = (double)(div[2] * m2);
+            const double v0 = M(bins, 0, 0) * c0 + M(bins, 0, 1) * c1 + M(bins, 0, 2) * c2;
+            const double v1 = M(bins, 1, 0) * c0 + M(bins, 1, 1) * c1 + M(bins, 1, 2) * c2;
+            const double v2 = M(bins, 2, 0) * c0 + M(bins, 2, 1) * c1 + M(bins, 2, 2) * c2;
+            const double d0 = e0 + v0, d1 = e1 + v1, d2 = e2 + v2;
+            const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+            if (dd > rr) continue;
+            const int bin = (int)(sqrt(dd) * inv_dr) + 1;
+            if (bin < 1 || bin > nbins) continue;
+            mine[((size_t)ta * ntypes + tb) * nbins + (bin - 1)] += 1;
+          }
+    }
+  }
+  for (size_t k = 0; k < nh; ++k) {
+    uint64_t s = 0;
+    for (int t = 0; t < nthr; ++t) s += acc[nh * (size_t)t + k];
+    hist[k] += s;
+  }
+  free(acc);
+  free(frac);
+  return 0;
+}

@@ -64,6 +64,23 @@ class DynInfo(_c.Structure):
     ]
 
 
+class DebyeConfig(_c.Structure):
+    _fields_ = [
+        ("abi_version", _c.c_int32), ("device", _c.c_int32), ("natoms", _c.c_int64),
+        ("ntypes", _c.c_int32), ("pbc", _c.c_int32 * 3), ("type_id", _c.POINTER(_c.c_int32)),
+        ("cell", _c.c_double * 9), ("origin", _c.c_double * 3), ("rmax", _c.c_double),
+        ("dr", _c.c_double), ("skin", _c.c_double),
+    ]
+
+
+class DebyeInfo(_c.Structure):
+    _fields_ = [
+        ("nbins", _c.c_int32), ("ntypes", _c.c_int32), ("div", _c.c_int32 * 3),
+        ("hist_in_lds", _c.c_int32), ("workgroups", _c.c_int32), ("reserved", _c.c_int32),
+        ("pairs_per_frame", _c.c_int64), ("device_bytes", _c.c_int64), ("nframes", _c.c_int64),
+    ]
+
+
 class SpeciesInfo(_c.Structure):
     _fields_ = [
         ("label", _c.c_char * 16), ("symbol", _c.c_char * 4), ("z", _c.c_int32),
@@ -112,6 +129,13 @@ SIGNATURES = {
     "sqc_hip_dyn_profile": (_c.c_int, [_vp, _c.c_int]),
     "sqc_hip_dyn_stage_times": (_c.c_int, [_vp, _vp, _vp, _c.c_int]),
     "sqc_hip_dyn_destroy": (_c.c_int, [_vp]),
+    "sqc_hip_debye_plan_create": (_c.c_int, [_c.POINTER(_vp), _c.POINTER(DebyeConfig)]),
+    "sqc_hip_debye_plan_info": (_c.c_int, [_vp, _c.POINTER(DebyeInfo)]),
+    "sqc_hip_debye_accumulate_device": (_c.c_int, [_vp, _vp, _c.c_int64]),
+    "sqc_hip_debye_accumulate": (_c.c_int, [_vp, _vp, _c.c_int64]),
+    "sqc_hip_debye_fetch": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_int64)]),
+    "sqc_hip_debye_reset": (_c.c_int, [_vp]),
+    "sqc_hip_debye_destroy": (_c.c_int, [_vp]),
     "sqc_hip_species_lookup": (_c.c_int, [_c.c_char_p, _c.POINTER(SpeciesInfo)]),
     "sqc_hip_species_table": (_c.c_int, [_c.c_int32, _c.c_int32, _c.POINTER(SpeciesInfo),
                                          _c.POINTER(_c.c_int32)]),

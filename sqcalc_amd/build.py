@@ -34,6 +34,9 @@ def build_library(verbose=False, jobs=2):
         # the matrix-core accumulators stay in VGPRs (no AGPR copies per MFMA)
         ("sqc_dyn.hip", ["-x", "hip", "--offload-arch=" + ARCH, "-mllvm", "-amdgpu-mfma-vgpr-form"]
          + DEVICE_FLAGS),
+        # pair displacements are evaluated in the reference's operation order
+        ("sqc_debye.hip", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off"]
+         + DEVICE_FLAGS),
         ("sqc_dump.cpp", ["-x", "c++", "-ffp-contract=off", "-pthread"]),
         ("sqc_elements.cpp", ["-x", "c++"]),
     ]

@@ -1468,6 +1468,7 @@ const char *sqc_hip_error_string(int code) {
     case SQC_HIP_ERR_NO_DEVICE: return "no usable HIP device";
     case SQC_HIP_ERR_HIP: return "HIP runtime error";
     case SQC_HIP_ERR_NO_MODES: return "no reciprocal lattice points in the q range";
+    case SQC_HIP_ERR_POSITIONS: return "atom positions outside what the pair list holds";
     default: return "unknown error";
   }
 }
