@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_F64_PEAK_TFLOPS = 78.6
+# MI355X vector FP64: 256 CUs x 64 fused multiply-adds (2 flops) per clock x 2.4 GHz
+FP64_VALU_PEAK_TFLOPS = 78.6
 COPY_BYTES = 1 << 30
 GRAD_EPS_F32 = 1e-7
 ERR_SHELLS = 40          # q < 0.8 at dq = 0.02: the oracle's direct sum finishes in seconds
@@ -57,6 +59,8 @@ def main():
     ap.add_argument("--ingest-frames", type=int, default=64)
     ap.add_argument("--no-dyn", action="store_true",
                     help="skip the dyn amplitude leg (96k atoms, lattice modes to |q| = 2)")
+    ap.add_argument("--no-debye", action="store_true",
+                    help="skip the Debye pair-histogram leg (96k atoms, rmax = L/2)")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the per-column error report against the FP64 oracle")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -234,6 +238,11 @@ def main():
     if not args.no_dyn:
         dyn = _dyn_leg(wl, host, dev, not args.no_cpu)
 
+    # ---- Debye: pair-distance histograms per type pair ----
+    debye = None
+    if not args.no_debye:
+        debye = _debye_leg(wl, host, dev, not args.no_cpu)
+
     # ---- per-column error of both precisions against the FP64 oracle ----
     errors = None
     if tables:
@@ -277,6 +286,7 @@ def main():
         "fwd_adj_grad_err_vs_fp64": grad_err,
         "column_max_abs_err_vs_fp64_oracle": errors,
         "dyn": dyn,
+        "debye": debye,
         "batched_trajectory": c3,
         "cpu_baseline": cpu,
         "ingest": ingest,
@@ -542,6 +552,76 @@ def _dyn_leg(wl, host, dev, with_cpu, qmax=2.0, fps=10, steps=5):
                                "sample": "all %d atoms x %d of %d modes in %.2f s (separable "
                                          "phase tables, src/sqc_dynamics.f90:826-851), scaled "
                                          "to all modes" % (wl.natoms, len(pick), len(hkl), dt)}
+    return res
+
+
+def _debye_leg(wl, host, dev, with_cpu, fps=4, steps=3, dr=0.01):
+    """Debye pair histograms of the 96,000-atom workload at the reference's
+    default cut-off (half the box side, 56.5 A, dr 0.01 A: 5650 bins x 3 type
+    pairs), FP64, positions resident: frames/s, the pair kernel's rate in
+    pairs/s and in FP64 operations against the vector FP64 peak (28 per pair:
+    the displacement pos_j - pos_i + bins . shift, its square, the root and
+    the scale to a bin, src/sqc_debye.f90:216-221), a full-size check (every
+    unordered pair once: counts symmetric, total near the ideal-gas
+    estimate), and the oracle's pair loop (the reference's arithmetic,
+    all host cores) on a sample of first atoms, scaled to the frame."""
+    import torch
+
+    from sqcalc_amd.debye import HipDebye
+    deb = HipDebye(wl.cell, wl.origin, wl.types, mapping=wl.mapping, rmax=None, dr=dr,
+                   weight=wl.weight)
+    info = deb.info
+    pos = torch.tensor(host[:fps], dtype=torch.float64, device=dev).contiguous()
+    deb.accumulate_device(pos[:1])
+    deb.histogram()
+    deb.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        deb.accumulate_device(pos)
+    h, nf = deb.histogram()
+    fr = fps * steps / (time.perf_counter() - t0)
+    deb.profile(True)
+    deb.accumulate_device(pos)
+    deb.histogram()
+    st = deb.stage_times()
+    deb.profile(False)
+    k_ms = st["pairs"][0] / st["pairs"][1]
+    n = wl.natoms
+    pairs = n * (n - 1) // 2
+    ops = 28.0 * pairs
+    vol = abs(float(np.linalg.det(wl.cell)))
+    ideal = n * (n - 1) * 4.0 / 3.0 * np.pi * deb.rmax ** 3 / vol
+    res = {"workload": "c2 geometry (96k atoms, L=113 A, 2 types 1:2), rmax = L/2 = %.2f A, dr "
+                       "%.3g A, %d bins x 3 type pairs, FP64" % (deb.rmax, dr, deb.nbins),
+           "frames_per_s": round(fr, 2), "pairs_per_frame": pairs,
+           "tile": int(info.tile), "workgroups": int(info.workgroups),
+           "hist_in_lds": int(info.hist_in_lds),
+           "stage_ms": {k: round(v[0] / v[1], 4) for k, v in st.items()},
+           "pairs_per_s": float("%.4g" % (pairs / (k_ms * 1e-3))),
+           "roofline": {"bound": "valu", "kernel": "k_debye_pairs", "unit": "TFLOP/s",
+                        "achieved": round(ops / (k_ms * 1e-3) / 1e12, 2),
+                        "peak": FP64_VALU_PEAK_TFLOPS,
+                        "frac": round(ops / (k_ms * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS, 4),
+                        "avg_ms": round(k_ms, 4), "ops_per_pair": 28},
+           "check": {"symmetric": bool(np.array_equal(h, h.transpose(1, 0, 2))),
+                     "ordered_pairs_per_frame": float(h.sum() / nf),
+                     "ideal_gas_estimate": float("%.6g" % ideal)}}
+    deb.close()
+    if with_cpu:
+        from oracle import oracle as O
+        run = O.DebyeRun(wl.cell, wl.origin, wl.types, int(np.max(wl.types)), rmax=deb.rmax,
+                         dr=dr)
+        rows = 8192
+        t1 = time.perf_counter()
+        run.rows(host[0], 0, rows)
+        dt = time.perf_counter() - t1
+        per_frame = dt * n / rows
+        res["cpu_baseline"] = {"value": round(1.0 / per_frame, 5), "unit": "frames/s",
+                               "cores": O.lib().sqo_num_threads(), "kind": "port",
+                               "sample": "%d of %d first atoms against all atoms in %.2f s (the "
+                                         "pair loop of debye_accumulate over every image within "
+                                         "rmax), scaled to the frame" % (rows, n, dt)}
     return res
 
 

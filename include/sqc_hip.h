@@ -304,7 +304,7 @@ typedef struct sqc_hip_debye_info {
   int32_t div[3];           /* the reference list's cells per cell vector */
   int32_t hist_in_lds;      /* 1: each workgroup counts into an LDS histogram */
   int32_t workgroups;
-  int32_t reserved;
+  int32_t tile;             /* atoms per tile = threads per workgroup (256 or 1024) */
   int64_t pairs_per_frame;  /* unordered atom pairs evaluated per frame */
   int64_t device_bytes;
   int64_t nframes;          /* frames counted since create / reset */
@@ -312,11 +312,11 @@ typedef struct sqc_hip_debye_info {
 
 int sqc_hip_debye_plan_create(sqc_hip_debye_plan **plan, const sqc_hip_debye_config *cfg);
 int sqc_hip_debye_plan_info(const sqc_hip_debye_plan *plan, sqc_hip_debye_info *info);
-/* Count nframes frames of device positions ([f][natoms][3]) on the plan's
- * stream; positions are read before the call's work ends (synchronise
- * with sqc_hip_debye_fetch before reusing them). */
+/* Count nframes frames of device positions ([f][natoms][3]), ordered on
+ * `stream` like sqc_hip_forward_device: the plan's work starts after what
+ * was queued on it and work queued on it afterwards waits for the counts. */
 int sqc_hip_debye_accumulate_device(sqc_hip_debye_plan *plan, const double *pos_device,
-                                    int64_t nframes);
+                                    int64_t nframes, void *stream);
 /* The same from host memory (synchronous). */
 int sqc_hip_debye_accumulate(sqc_hip_debye_plan *plan, const double *pos, int64_t nframes);
 /* histogram[a][b][k] (ntypes x ntypes x nbins, C order, 0-based) of ordered
@@ -332,6 +332,11 @@ int sqc_hip_debye_fetch(sqc_hip_debye_plan *plan, double *histogram, int64_t *nf
 /* Clear the counts, the frame count and the position flags (the list's
  * unwrapped positions carry on, as the reference's do). */
 int sqc_hip_debye_reset(sqc_hip_debye_plan *plan);
+/* HIP-event times on the plan stream: 0 list positions, 1 pair counts,
+ * 2 row sum (as sqc_hip_profile / sqc_hip_stage_times). */
+int sqc_hip_debye_profile(sqc_hip_debye_plan *plan, int enable);
+int sqc_hip_debye_stage_times(sqc_hip_debye_plan *plan, double *ms, int64_t *launches,
+                              int nstage);
 int sqc_hip_debye_destroy(sqc_hip_debye_plan *plan);
 
 /* ---- Species labels and their scattering data (host side, no device) ----

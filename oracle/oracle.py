@@ -101,6 +101,10 @@ def lib():
     L.sqo_cell_widths.argtypes = [_d, _d]
     L.sqo_debye_cells.argtypes = [_d, ctypes.c_double, ctypes.c_double, _i32, _d]
     L.sqo_debye_positions.argtypes = [ctypes.c_int64, _d, _d, _d, ctypes.c_int, _d, _d]
+    L.sqo_debye_hist_rows.restype = ctypes.c_int
+    L.sqo_debye_hist_rows.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                      _i32, _d, _i32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, _d, ctypes.c_int, _u64]
     L.sqo_debye_hist.restype = ctypes.c_int
     L.sqo_debye_hist.argtypes = [ctypes.c_int64, ctypes.c_int, _i32, _d, _i32, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, _d, ctypes.c_int, _u64]
@@ -513,6 +517,23 @@ class DebyeRun:
         self.posw = np.zeros((n, 3))
         self.nframes = 0
         self.hist = np.zeros((self.ntypes, self.ntypes, self.nbins), dtype=np.uint64)
+
+    def rows(self, r, i0, i1, method=1):
+        """Counts of the pairs whose first atom is in [i0, i1) of a first
+        frame r (the list positions of that frame), not added to the run."""
+        r = _f64(np.asarray(r).reshape(-1, 3))
+        pos, posw = np.zeros_like(r), np.zeros_like(r)
+        L = lib()
+        L.sqo_debye_positions(len(self.types), _p(self.a, _d), _p(self.origin, _d), _p(r, _d), 1,
+                              _p(pos, _d), _p(posw, _d))
+        h = np.zeros_like(self.hist)
+        rc = L.sqo_debye_hist_rows(len(self.types), i0, i1, self.ntypes, _p(self.types, _i32),
+                                   _p(self.a, _d), _p(self.pbc, _i32), self.rmax, self.dr,
+                                   self.skin, _p(pos, _d), method,
+                                   h.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        if rc:
+            raise MemoryError("oracle debye_hist")
+        return h
 
     def add_frame(self, r, method=1):
         r = _f64(np.asarray(r).reshape(-1, 3))

@@ -654,9 +654,9 @@ void sqo_debye_positions(int64_t natoms, const double *a, const double *origin, 
  * from a box of +-(ceil(rmax / width_k) + 1) around the minimum image (brute
  * force).  hist[a][b][k] (0-based, ntypes x ntypes x nbins) gains the counts;
  * type_id is 1-based. */
-int sqo_debye_hist(int64_t natoms, int ntypes, const int32_t *type_id, const double *a,
-                   const int32_t *pbc, double rmax, double dr, double skin, const double *pos,
-                   int method, uint64_t *hist) {
+int sqo_debye_hist_rows(int64_t natoms, int64_t i0, int64_t i1, int ntypes,
+                        const int32_t *type_id, const double *a, const int32_t *pbc, double rmax,
+                        double dr, double skin, const double *pos, int method, uint64_t *hist) {
   double inv[9], w[3], bins[9];
   int32_t div[3];
   sqo_inverse3(a, inv);
@@ -685,7 +685,7 @@ int sqo_debye_hist(int64_t natoms, int ntypes, const int32_t *type_id, const dou
     wide[k] = (int)ceil(rmax / w[k]) + 1;
   }
 #pragma omp parallel for schedule(dynamic, 16)
-  for (int64_t i = 0; i < natoms; ++i) {
+  for (int64_t i = i0; i < i1; ++i) {
 #ifdef _OPENMP
     uint64_t *mine = acc + nh * (size_t)omp_get_thread_num();
 #else
@@ -736,4 +736,13 @@ int sqo_debye_hist(int64_t natoms, int ntypes, const int32_t *type_id, const dou
   free(acc);
   free(frac);
   return 0;
+}
+
+/* All first atoms i of the frame (sqo_debye_hist_rows takes the pairs whose
+ * first atom is in [i0, i1), which the CPU timing samples). */
+int sqo_debye_hist(int64_t natoms, int ntypes, const int32_t *type_id, const double *a,
+                   const int32_t *pbc, double rmax, double dr, double skin, const double *pos,
+                   int method, uint64_t *hist) {
+  return sqo_debye_hist_rows(natoms, 0, natoms, ntypes, type_id, a, pbc, rmax, dr, skin, pos,
+                             method, hist);
 }

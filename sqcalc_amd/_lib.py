@@ -76,7 +76,7 @@ class DebyeConfig(_c.Structure):
 class DebyeInfo(_c.Structure):
     _fields_ = [
         ("nbins", _c.c_int32), ("ntypes", _c.c_int32), ("div", _c.c_int32 * 3),
-        ("hist_in_lds", _c.c_int32), ("workgroups", _c.c_int32), ("reserved", _c.c_int32),
+        ("hist_in_lds", _c.c_int32), ("workgroups", _c.c_int32), ("tile", _c.c_int32),
         ("pairs_per_frame", _c.c_int64), ("device_bytes", _c.c_int64), ("nframes", _c.c_int64),
     ]
 
@@ -131,7 +131,9 @@ SIGNATURES = {
     "sqc_hip_dyn_destroy": (_c.c_int, [_vp]),
     "sqc_hip_debye_plan_create": (_c.c_int, [_c.POINTER(_vp), _c.POINTER(DebyeConfig)]),
     "sqc_hip_debye_plan_info": (_c.c_int, [_vp, _c.POINTER(DebyeInfo)]),
-    "sqc_hip_debye_accumulate_device": (_c.c_int, [_vp, _vp, _c.c_int64]),
+    "sqc_hip_debye_accumulate_device": (_c.c_int, [_vp, _vp, _c.c_int64, _vp]),
+    "sqc_hip_debye_profile": (_c.c_int, [_vp, _c.c_int]),
+    "sqc_hip_debye_stage_times": (_c.c_int, [_vp, _vp, _vp, _c.c_int]),
     "sqc_hip_debye_accumulate": (_c.c_int, [_vp, _vp, _c.c_int64]),
     "sqc_hip_debye_fetch": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_int64)]),
     "sqc_hip_debye_reset": (_c.c_int, [_vp]),

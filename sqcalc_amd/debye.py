@@ -202,8 +202,20 @@ class HipDebye:
         nf = 1 if t.dim() == 2 else t.shape[0]
         if t.numel() != nf * self.natoms * 3:
             raise ValueError("positions do not match natoms")
-        _lib.check(self.lib.sqc_hip_debye_accumulate_device(self.plan, t.data_ptr(), nf),
+        import torch
+        stream = torch.cuda.current_stream(t.device).cuda_stream
+        _lib.check(self.lib.sqc_hip_debye_accumulate_device(self.plan, t.data_ptr(), nf, stream),
                    "debye_accumulate_device")
+
+    def profile(self, enable=True):
+        _lib.check(self.lib.sqc_hip_debye_profile(self.plan, 1 if enable else 0), "debye_profile")
+
+    def stage_times(self):
+        """{stage: (total ms, launches)} since the last call: positions, pairs, row_sum."""
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_int64 * 3)()
+        _lib.check(self.lib.sqc_hip_debye_stage_times(self.plan, ms, n, 3), "debye_stage_times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(("positions", "pairs", "row_sum"))}
 
     def histogram(self):
         """(hist[a][b][k] ordered pair counts summed over the frames, nframes)."""

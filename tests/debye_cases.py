@@ -47,13 +47,24 @@ CASES = {
     "small_box": ((8.0, 8.0, 8.0), (0, 0, 0), (0, 0, 0), (1, 1, 1), 120, 5.0, 0.5, 3, 0.4),
     "skin_reuse": ((20.0, 20.0, 20.0), (0, 0, 0), (0, 0, 0), (1, 1, 1), 400, 5.0, 1.0, 20, 0.02),
     "wide_cut": ((20.0, 21.0, 22.0), (3.0, 0, -2.0), (0, 0, 0), (1, 1, 1), 400, 9.9, 0.0, 2, 1.5),
+    # a general cell (no zero entries): the triclinic one turned about (1, 2, 2) / 3
+    "rotated": ((18.0, 17.0, 21.0), (2.5, -1.5, 3.0), (0.5, 0.5, -1.0), (1, 1, 1), 300, 6.0, 1.0,
+                3, 0.3),
 }
+
+
+def rotation(axis, angle):
+    axis = np.asarray(axis, dtype=np.float64) / np.linalg.norm(axis)
+    k = np.array([[0.0, -axis[2], axis[1]], [axis[2], 0.0, -axis[0]], [-axis[1], axis[0], 0.0]])
+    return np.eye(3) + np.sin(angle) * k + (1.0 - np.cos(angle)) * (k @ k)
 
 
 def run_case(name, seed=3, ntypes=3):
     lengths, tilt, origin, pbc, n, rmax, skin, nf, sigma = CASES[name]
     rng = np.random.default_rng(seed)
     cell = make_cell(lengths, tilt)
+    if name == "rotated":
+        cell = rotation((1.0, 2.0, 2.0), 0.7) @ cell
     origin = np.asarray(origin, dtype=np.float64)
     frames = trajectory(rng, cell, origin, n, nf, sigma, pbc)
     types = rng.integers(1, ntypes + 1, n).astype(np.int32)

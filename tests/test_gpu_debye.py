@@ -100,20 +100,20 @@ def test_large_bit_exact(hip_device, n):
 
 
 def test_histogram_outside_lds(hip_device):
-    """A histogram too large for the LDS (8 types, fine bins): the shared-row
-    path with HBM atomics."""
+    """A histogram row too long for the LDS (28,000 bins, 8 types): the
+    shared-row path with HBM atomics."""
     from sqcalc_amd.debye import HipDebye
     print("case debye hbm histogram", flush=True)
     rng = np.random.default_rng(8)
     cell = make_cell((30.0, 31.0, 29.0), (1.0, -2.0, 0.5))
     frames = trajectory(rng, cell, np.zeros(3), 2500, 2, 0.5)
     types = rng.integers(1, 9, 2500).astype(np.int32)
-    deb = HipDebye(cell, np.zeros(3), types, rmax=14.0, dr=0.002)
+    deb = HipDebye(cell, np.zeros(3), types, rmax=14.0, dr=0.0005)
     assert deb.info.hist_in_lds == 0
     for f in frames:
         deb.accumulate(f)
     h, _ = deb.histogram()
-    run = oracle_hist(cell, np.zeros(3), (1, 1, 1), types, 8, 14.0, 0.002, 1.0, frames)
+    run = oracle_hist(cell, np.zeros(3), (1, 1, 1), types, 8, 14.0, 0.0005, 1.0, frames)
     assert np.array_equal(h, run.hist.astype(np.float64))
     deb.close()
 
