@@ -16,14 +16,19 @@
 //
 // On the device the pairs are counted unordered: d(j, i) = -d(i, j) exactly
 // (IEEE subtraction and products are sign-symmetric), so one evaluation adds
-// the pair to (t_i, t_j) and (t_j, t_i).  Persistent workgroups walk the
-// upper triangle of 256 x 256 atom tiles; the j tile sits in LDS, each
-// thread takes one i.  For every pair the images m with |s_k + m_k| within
-// rmax / width_k (s the fractional displacement) are tried, so cut-offs up
-// to the smallest width are counted like the cell list counts them.  Counts
-// go to a u32 histogram in LDS (integer adds: the result does not depend
-// on their order), or straight to the workgroup's row in HBM when it does
-// not fit; the rows are summed into the run's u64 histogram per frame.
+// the pair to (t_i, t_j) and (t_j, t_i).  Atoms sit in slots sorted by type,
+// each type padded to whole tiles of 1024 (or 256) slots, so a pair of tiles
+// is one type pair.  Persistent workgroups walk contiguous runs of the upper
+// triangle of tile pairs; the j tile sits in LDS, each thread takes one i.
+// A pair first tries its nearest image m = -rint(s) (s the fractional
+// displacement); only when a second image can be within reach (rmax above
+// half a cell width) are all images m with |s_k + m_k| within rmax / width_k
+// tried, so cut-offs up to the smallest width are counted like the cell
+// list counts them.  Counts go to one u32 histogram row in LDS (integer
+// adds: the result does not depend on their order), added to the
+// workgroup's own row in HBM when the type pair changes, or straight to HBM
+// rows when a row does not fit the LDS; the rows are summed into the run's
+// u64 histogram per frame.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

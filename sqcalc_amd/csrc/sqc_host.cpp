@@ -287,6 +287,52 @@ double amplitude(const sqc::Geo &g, int sp, double q) {
   return g.amp[sp];
 }
 
+// Quanta of the exact shell sums (the x pass's exact_parts).  A value added
+// into a shell is 2 |sum_a w_a rho_a|^2 (the total) or 2 ps Re(rho_a conj
+// rho_b) (a pair), and |rho_a| <= N_a for a field of unit strengths
+// (|rho| <= sum_a |w_a| N_a for one weighted field), so |v| <= vmax below,
+// with a factor 4 for the transform's own error; a shell takes at most cmax
+// values per component in a frame.  q = 2^e with cmax vmax / q <= 2^62 keeps
+// every sum of hi words (each at most vmax / q + 1/2) below 2^63, and
+// vmax / q <= 2^50 keeps each within the range of the rounding step; the lo
+// words are at most 2^(k-1) each, and 2^k <= 2^62 / 2^ceil(log2(cmax + 1))
+// (and <= 2^51) keeps their sums below 2^61.  One word carries a value to
+// q / 2, about
+// 1e-8 of a typical mode at 96,000 atoms, where vmax is the Bragg bound N^2
+// times the weights: FP32 plans (whose transform is good to 1e-7) keep one,
+// FP64 plans two (q 2^-k / 2, far below the FP64 rounding of the sums).
+int ceil_log2(double x) { return (int)std::ceil(std::log2(x)); }
+
+void set_shell_quanta(sqc::Plan &p, double qmax) {
+  sqc::Geo &g = p.geo;
+  const int S = g.S;
+  const double smax = qmax / (4.0 * std::acos(-1.0));
+  double A = 0.0, nn = 0.0;
+  for (int a = 0; a < S; ++a) {
+    double w;
+    if (g.weight_kind == SQC_HIP_WEIGHT_XRAY) {
+      // |f_a(q)| <= sum_i |a_i| max(1, exp(-b_i s^2)) + |c| over the q range
+      const double *c = g.xcoef[a];
+      w = std::fabs(c[8]);
+      for (int i = 0; i < 4; ++i)
+        w += std::fabs(c[i]) * std::max(1.0, std::exp(-c[4 + i] * smax * smax));
+    } else {
+      w = std::fabs(g.amp[a]);
+    }
+    A += w * (double)p.counts[a];
+    for (int b = a; b < S; ++b) nn = std::max(nn, (double)p.counts[a] * (double)p.counts[b]);
+  }
+  int64_t cmax = 1;
+  for (int64_t c : p.shell_count) cmax = std::max(cmax, c);
+  const double tiny = std::ldexp(1.0, -900);
+  const double vmax[2] = {4.0 * 2.0 * A * A, 4.0 * 2.0 * nn};
+  for (int c = 0; c < 2; ++c)
+    g.xq_inv[c] = std::ldexp(1.0, std::min(62 - ceil_log2(std::max((double)cmax * vmax[c], tiny)),
+                                           50 - ceil_log2(std::max(vmax[c], tiny))));
+  g.xq_lo = std::ldexp(1.0, std::min(51, 62 - ceil_log2((double)cmax + 1.0)));
+  g.xwords = p.prec == 1 ? 1 : 2;
+}
+
 // Spread tiles (txr real x-columns, ty y-columns), largest first.  Every
 // tile has txr * ty <= 32, which keeps a stencil's xy footprint inside a
 // tile at 64 cells or fewer (the spreader's per-lane xy table relies on it),
@@ -404,9 +450,9 @@ int alloc_bins(sqc::Plan &p) {
 // accumulators and the 1 / phihat_1 table.
 size_t xpass_lds(const sqc::Plan &p, int lb) {
   const sqc::Geo &g = p.geo;
-  // shell sums (ncomp rows of nq) unless they live in HBM; the adjoint x
-  // pass keeps one row of nq there, two with pair gradients (ncomp >= 2)
-  const size_t rows = g.acc_global ? 2 : (size_t)g.ncomp;
+  // shell sums (the integer words of ncomp rows of nq) unless they live in
+  // HBM; the adjoint x pass keeps two rows of nq there
+  const size_t rows = g.acc_global ? 2 : std::max<size_t>(2, (size_t)g.xwords * g.ncomp);
   const size_t accb = (rows * g.nq + g.M[0] + 1) * sizeof(double);
   return ((size_t)lb * g.F * g.rs1 + g.n[0]) * vsize(p) + accb;
 }
@@ -424,14 +470,15 @@ void set_xpass(sqc::Plan &p, int lb) {
 
 // Per-workgroup shell rows of the x pass (one row per persistent workgroup).
 int alloc_wgpart(sqc::Plan &p) {
-  const size_t n = (size_t)p.geo.ncomp * p.geo.nq * p.nblk3;
+  const size_t row = (size_t)p.geo.xwords * p.geo.ncomp * p.geo.nq;
+  const size_t n = row * p.nblk3;
   if (p.d_wgpart && p.wgpart_rows == p.nblk3) return 0;
   if (p.d_wgpart) {
     (void)hipFree(p.d_wgpart);
-    p.bytes -= (int64_t)sizeof(double) * p.geo.ncomp * p.geo.nq * p.wgpart_rows;
+    p.bytes -= (int64_t)(sizeof(uint64_t) * row * p.wgpart_rows);
     p.d_wgpart = nullptr;
   }
-  if (int rc = alloc(&p.d_wgpart, sizeof(double) * n, p.bytes)) return rc;
+  if (int rc = alloc(&p.d_wgpart, sizeof(uint64_t) * n, p.bytes)) return rc;
   p.wgpart_rows = p.nblk3;
   return 0;
 }
@@ -561,6 +608,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   }
   if (p.nmodes == 0)
     return fail(SQC_HIP_ERR_NO_MODES, "no reciprocal lattice points fall inside the requested q range");
+  set_shell_quanta(p, cfg->qmax);
 
   // canonical half-space lines: h3 > 0; h3 = 0, h2 > 0; h3 = h2 = 0, h1 > 0
   const int M1 = g.M[0], M2 = g.M[1], M3 = g.M[2];

@@ -6,7 +6,8 @@
 //   W1       [F][M3+1][n2][n1]        T2    after spread + z-DFT, h3 >= 0 only
 //   W2       [F][nlines][n1]          T2    after the y-DFT, canonical (h2,h3) lines
 //   stab     [ncand]                  u16   shell of each candidate mode (0xFFFF: not kept)
-//   wgpart   [nblk][ncomp][nq]        f64   per-workgroup shell partials of one frame
+//   wgpart   [nblk][words][ncomp][nq] i64   per-workgroup shell partials of one frame
+//                                           (integer words of the exact sums)
 //   acc      [2][ncomp][nq]           f64   run accumulators (sum, compensation)
 // Real strengths make F[g] real, so only the half spectrum h3 > 0, or h3 = 0
 // and h2 > 0, or h3 = h2 = 0 and h1 > 0 is transformed; its modes count twice.
@@ -79,6 +80,11 @@ struct Geo {
   // 1: the x pass keeps its shell sums in its own row of the per-workgroup
   // partials in HBM (FP64 atomics) because ncomp * nq doubles exceed the LDS
   int acc_global;
+  // exact shell sums: 1 / q of the total and of the pair sums (q a power of
+  // two), 2^k of the remainder word, and the words per sum (1 or 2)
+  double xq_inv[2];
+  double xq_lo;
+  int xwords;
   double xcoef[kMaxSpecies][9];
   int nxcd;         // XCDs the workgroups are dealt to
 };
@@ -142,7 +148,8 @@ struct Plan {
   uint16_t *d_stab = nullptr;
   double *d_line_d23 = nullptr;  // per x line: 1 / (phihat_2 phihat_3)
   double *d_den = nullptr, *d_cnt = nullptr;
-  double *d_wgpart = nullptr, *d_acc = nullptr, *d_frame = nullptr;
+  unsigned long long *d_wgpart = nullptr;
+  double *d_acc = nullptr, *d_frame = nullptr;
   double *d_gnum = nullptr;
   double *d_gpart = nullptr;     // [npairs][ncand] per-mode pair sums (mode_partials)
   double *d_slots = nullptr;

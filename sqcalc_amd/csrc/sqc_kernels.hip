@@ -1093,6 +1093,38 @@ __device__ __forceinline__ void load_group(LineGroup &lg, int l0, int nl,
   gl.put(lg, nl);
 }
 
+// Exact shell sums.  Every value v added into a shell accumulator becomes a
+// 64-bit integer count of the plan's quantum q (a power of two): hi =
+// rint(v / q).  FP64 plans keep a second word for the remainder, lo =
+// rint((v / q - hi) 2^k).  The host picks q from a bound on |v| and on the
+// number of values per shell, and k from the latter, so that no sum of his
+// or of los of one shell can leave 63 bits: integer additions are exact, so
+// a shell's sums do not depend on the order in which lanes, waves or
+// workgroups add them (Demmel and Nguyen's reproducible summation rounds to
+// a common quantum the same way).  A value is then carried to q / 2 (one
+// word) or q 2^-k / 2 (two words).  The rounding to an integer is one FP64
+// add: for |x| < 2^51, x + 1.5 2^52 lies in [2^52, 2^53), where the spacing
+// is 1, so it is rint(x) plus the constant, and its bits minus the
+// constant's bits are rint(x) as an integer.  v / q, the remainder and its
+// scaling are exact (powers of two, |v / q - hi| <= 1/2); contraction is off
+// so that each rounding happens as written.
+struct Parts {
+  long long hi, lo;
+};
+__device__ __forceinline__ Parts exact_parts(double v, double inv_q, double lo_scale, bool two) {
+#pragma clang fp contract(off)
+  constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+  const long long mbits = __double_as_longlong(kMagic);
+  const double x = v * inv_q;
+  const double t = x + kMagic;
+  Parts p{__double_as_longlong(t) - mbits, 0};
+  if (two) {
+    const double y = (x - (t - kMagic)) * lo_scale;
+    p.lo = __double_as_longlong(y + kMagic) - mbits;
+  }
+  return p;
+}
+
 // FC: the number of fields when the epilogue is compiled for it (0: g.F at
 // run time).  With more than one field, fields and species coincide.  LB:
 // the lines per group when compiled in (2, the common tuned value; 0: lb_).
@@ -1105,7 +1137,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
                                                   const uint16_t *__restrict__ stab,
                                                   const double *__restrict__ ph1,
                                                   const double *__restrict__ line_d23,
-                                                  double *__restrict__ wgpart,
+                                                  unsigned long long *__restrict__ wgpart,
                                                   typename C2<T>::t *__restrict__ rho_out,
                                                   int64_t ncand, double *__restrict__ gnum,
                                                   double *__restrict__ gpart) {
@@ -1118,33 +1150,47 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
   const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
   V *tw = rows + (size_t)lb * F * rs;
-  // shell sums in LDS, or, when ncomp * nq doubles do not fit there, in this
-  // workgroup's own row of wgpart (zeroed here; the group loop's barrier
-  // orders that before the first add)
+  // exact shell sums (exact_parts): the hi words of the ncomp x nq sums,
+  // then (FP64 plans) their lo words, in LDS, or, when they do not fit
+  // there, in this workgroup's own row of wgpart (zeroed here; the group
+  // loop's barrier orders that before the first add)
   // (two pointers, not one selected at run time, so that the LDS adds stay
   // LDS instructions)
   // only the run-time field-count instance keeps sums in HBM (the host
   // routes every such plan there), so the compiled instances keep pure LDS adds
   const bool gacc = FC == 0 && g.acc_global != 0;
-  double *acc = reinterpret_cast<double *>(tw + n1);
-  double *accg = wgpart + (size_t)blockIdx.x * ncomp * nq;
-  double *inv1 = gacc ? acc : acc + (size_t)ncomp * nq;  // 1 / phihat_1(|h1|)
+  const int ntot = ncomp * nq, nw = g.xwords * ntot;
+  unsigned long long *acc = reinterpret_cast<unsigned long long *>(tw + n1);
+  unsigned long long *accg = wgpart + (size_t)blockIdx.x * nw;
+  // 1 / phihat_1(|h1|)
+  double *inv1 = reinterpret_cast<double *>(gacc ? acc : acc + nw);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   if (gacc)
-    for (int i = tid; i < ncomp * nq; i += NT) accg[i] = 0.0;
+    for (int i = tid; i < nw; i += NT) accg[i] = 0ull;
   else
-    for (int i = tid; i < ncomp * nq; i += NT) acc[i] = 0.0;
+    for (int i = tid; i < nw; i += NT) acc[i] = 0ull;
   // address spaces spelled out: the two adds must not merge into one generic
   // (flat) atomic on a selected pointer
-  typedef __attribute__((address_space(3))) double lds_double;
-  typedef __attribute__((address_space(1))) double hbm_double;
+  typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+  typedef __attribute__((address_space(1))) unsigned long long hbm_u64;
+  // quantum of the total (component 0) and of the pair sums
+  const double iq0 = g.xq_inv[0], iq1 = g.xq_inv[1], los = g.xq_lo;
+  const bool two = g.xwords == 2;
   auto add = [&](int i, double v) {
-    if (gacc)
-      __hip_atomic_fetch_add((hbm_double *)(accg + i), v, __ATOMIC_RELAXED,
+    const Parts pt = exact_parts(v, i < nq ? iq0 : iq1, los, two);
+    if (gacc) {
+      __hip_atomic_fetch_add((hbm_u64 *)(accg + i), (unsigned long long)pt.hi, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
-    else
-      __hip_atomic_fetch_add((lds_double *)(acc + i), v, __ATOMIC_RELAXED,
+      if (two)
+        __hip_atomic_fetch_add((hbm_u64 *)(accg + ntot + i), (unsigned long long)pt.lo,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_fetch_add((lds_u64 *)(acc + i), (unsigned long long)pt.hi, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (two)
+        __hip_atomic_fetch_add((lds_u64 *)(acc + ntot + i), (unsigned long long)pt.lo,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
   };
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
   for (int i = tid; i <= M1; i += NT) inv1[i] = 1.0 / ph1[i];
@@ -1253,12 +1299,14 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
   }
   __syncthreads();
   if (gacc) return;  // the sums are already in the row
-  double *out = wgpart + (size_t)blockIdx.x * ncomp * nq;
-  for (int i = tid; i < ncomp * nq; i += NT) out[i] = acc[i];
+  for (int i = tid; i < nw; i += NT) accg[i] = acc[i];
 }
 
-// Ordered, compensated (Neumaier) sum over the workgroup partials: each entry
-// gets kFinParts strided partial sums that are combined in a fixed order.
+// Sum of the workgroup rows of one frame: the integer words of an entry are
+// summed exactly (exact_parts bounds them), so the order of the rows does
+// not matter; the frame's value is hi q + lo q 2^-k.  The frame is then
+// added into the run's accumulators with Neumaier's compensation, frame
+// after frame.
 constexpr int kFinParts = 16;
 constexpr int kFinEntries = 256 / kFinParts;
 
@@ -1268,37 +1316,44 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double v) {
   s = t;
 }
 
-__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ wgpart, int nblk,
-                                                  int ntot, double *__restrict__ acc,
+__global__ void __launch_bounds__(256) k_finalize(const unsigned long long *__restrict__ wgpart,
+                                                  int nblk, int ntot, int words, int nq,
+                                                  double q0, double q1, double lo_q,
+                                                  double *__restrict__ acc,
                                                   double *__restrict__ frame_out, int accumulate) {
-  __shared__ double ps[kFinParts][kFinEntries], pc[kFinParts][kFinEntries];
+  __shared__ unsigned long long ph[kFinParts][kFinEntries], pl[kFinParts][kFinEntries];
   const int e = threadIdx.x % kFinEntries, part = threadIdx.x / kFinEntries;
   const int i = blockIdx.x * kFinEntries + e;
-  double s = 0.0, c = 0.0;
+  const size_t row = (size_t)words * ntot;
+  unsigned long long sh = 0, sl = 0;
   if (i < ntot) {
     int b = part;
     for (; b + 3 * kFinParts < nblk; b += 4 * kFinParts) {
-      const double v0 = wgpart[(size_t)b * ntot + i];
-      const double v1 = wgpart[(size_t)(b + kFinParts) * ntot + i];
-      const double v2 = wgpart[(size_t)(b + 2 * kFinParts) * ntot + i];
-      const double v3 = wgpart[(size_t)(b + 3 * kFinParts) * ntot + i];
-      neumaier(s, c, v0);
-      neumaier(s, c, v1);
-      neumaier(s, c, v2);
-      neumaier(s, c, v3);
+      unsigned long long h[4], l[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        h[u] = wgpart[(size_t)(b + u * kFinParts) * row + i];
+        if (words == 2) l[u] = wgpart[(size_t)(b + u * kFinParts) * row + ntot + i];
+      }
+      sh += (h[0] + h[1]) + (h[2] + h[3]);
+      sl += (l[0] + l[1]) + (l[2] + l[3]);
     }
-    for (; b < nblk; b += kFinParts) neumaier(s, c, wgpart[(size_t)b * ntot + i]);
+    for (; b < nblk; b += kFinParts) {
+      sh += wgpart[(size_t)b * row + i];
+      if (words == 2) sl += wgpart[(size_t)b * row + ntot + i];
+    }
   }
-  ps[part][e] = s;
-  pc[part][e] = c;
+  ph[part][e] = sh;
+  pl[part][e] = sl;
   __syncthreads();
   if (part != 0 || i >= ntot) return;
-  double ts = 0.0, tc = 0.0;
+  unsigned long long th = 0, tl = 0;
   for (int k = 0; k < kFinParts; ++k) {
-    neumaier(ts, tc, ps[k][e]);
-    tc += pc[k][e];
+    th += ph[k][e];
+    tl += pl[k][e];
   }
-  const double total = ts + tc;
+  const double q = i < nq ? q0 : q1;
+  const double total = (double)(long long)th * q + (double)(long long)tl * (q * lo_q);
   if (frame_out) frame_out[i] = total;
   if (accumulate) {
     double a = acc[i], cc = acc[ntot + i];
@@ -1340,7 +1395,7 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
   double *inv1 = reinterpret_cast<double *>(tw + n1);
   // per shell: G = gS / den (0 for an empty shell) and, with pair gradients,
   // 1 / (N max(count, 1)), in the LDS the forward pass keeps its shell sums
-  // in (ncomp >= 2 rows of nq when there are pair gradients)
+  // in (two rows of nq)
   double *Gsh = inv1 + g.M[0] + 1, *inc = Gsh + nq;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
@@ -1923,9 +1978,10 @@ static int forward_x_t(Plan &p, const void *W2, hipStream_t st, int mode, double
 int launch_finalize(Plan &p, double *frame_out, int accumulate, hipStream_t st) {
   const int ntot = p.geo.ncomp * p.geo.nq;
   StageTimer t(p, 4);
+  const Geo &g = p.geo;
   hipLaunchKernelGGL(k_finalize, dim3((ntot + kFinEntries - 1) / kFinEntries), dim3(256), 0,
-                     st, p.d_wgpart,
-                     p.nblk3, ntot, p.d_acc, frame_out, accumulate);
+                     st, p.d_wgpart, p.nblk3, ntot, g.xwords, g.nq, 1.0 / g.xq_inv[0],
+                     1.0 / g.xq_inv[1], 1.0 / g.xq_lo, p.d_acc, frame_out, accumulate);
   SQC_CHECK(hipGetLastError());
   return 0;
 }

@@ -11,7 +11,8 @@ c2  L=113, N=96,000, neutron, partials, q_max 10:
       same), 1e-9;
     - FP32 against FP64 on every column, 1e-6;
     - the shell sums against the pair sums: num = sum_{a<=b} (2 - d_ab) b_a b_b
-      P_ab holds mode by mode, so to FP64 rounding;
+      P_ab holds mode by mode, so to the rounding of the shell sums (FP64
+      plans 1e-12, FP32 plans' one-word sums 1e-9);
     - the perfect 46^3 lattice: S of a shell is N * (Bragg modes in the
       shell) / (modes in the shell), every other mode giving exactly zero.
 c3  L=80, N=32,000, 60 frames: graph replays against kernel-by-kernel
@@ -118,7 +119,11 @@ def test_c2_single_against_double_and_sum_rule(hip_device, c2_frame):
                 recon += (1.0 if a == c else 2.0) * b[a] * b[c] * part[a, c]
         err = np.abs(recon - num).max() / np.abs(num).max()
         print("  %s: sum rule %.2e" % (precision, err))
-        assert err < 1e-12
+        # num and the pair sums are exact integer sums of each mode's value
+        # rounded to the plan's quantum: two words (FP64 plans) carry a value
+        # far below FP64 rounding, one word (FP32 plans) to about 1e-8 of a
+        # typical mode here
+        assert err < (1e-12 if precision == "double" else 1e-9)
 
 
 @pytest.mark.parametrize("precision,tol", [("double", 1e-9), ("single", 1e-6)])

@@ -127,12 +127,13 @@ def test_dense_windows_span_many_staging_rounds(hip_device, precision):
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-@pytest.mark.parametrize("nq,in_hbm", [(500, 0), (600, 1)])
+@pytest.mark.parametrize("nq,in_hbm", [(250, 0), (600, 1)])
 def test_eight_species_shell_sums(hip_device, precision, nq, in_hbm):
-    """8 species with partials: 37 rows of FP64 shell sums.  At 500 shells
-    (148 KB) they still fit the LDS beside one x line; at 600 shells
-    (178 KB) they do not, and the x pass adds them in its own row of the
-    per-workgroup partials in HBM (the plan reports which).  Every column
+    """8 species with partials: 37 rows of exact FP64 shell sums, each a hi
+    and a lo part.  At 250 shells (148 KB) they still fit the LDS beside one
+    x line; at 600 shells (355 KB) they do not, and the x pass adds them in
+    its own row of the per-workgroup partials in HBM (the plan reports
+    which).  Every column
     against the oracle (eps 1e-10 in FP64 and 1e-6 in FP32: with 60 atoms a
     species the pair columns are small, and this case checks the
     accumulation path, not the FP32 default's accuracy, which the golden
