@@ -6,6 +6,12 @@
       (the ``*_kernel_stats.csv`` file), and with --trace the same
       dispatches grouped by launch shape.
 
+  python -m sqcalc_amd.rocprof_summary counters CSV [CSV ...] --command "..."
+      per-kernel averages of every counter or derived metric in one or more
+      ``rocprofv3 --pmc`` runs (``*_counter_collection.csv``), each kernel
+      in its most frequent launch shape, with VALU and LDS instructions per
+      wave beside them.
+
   python -m sqcalc_amd.rocprof_summary traffic --fetch FETCH_CSV --write WRITE_CSV
          --copy-bytes B --json out.json --command "..." > out.md
       HBM bytes per dispatch from two ``rocprofv3 --pmc FETCH_SIZE`` and
@@ -177,6 +183,48 @@ def cmd_traffic(args):
             fh.write("\n")
 
 
+def read_counters(paths):
+    """{kernel: {counter: (mean per dispatch, dispatches)}} over every counter
+    in the files, each kernel in its most frequent launch shape."""
+    acc = collections.defaultdict(lambda: collections.defaultdict(
+        lambda: collections.defaultdict(list)))
+    for path in paths:
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                shape = (r.get("Grid_Size"), r.get("LDS_Block_Size"))
+                acc[r["Kernel_Name"]][shape][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = {}
+    for k, shapes in acc.items():
+        best = max(shapes.values(), key=lambda c: max(len(v) for v in c.values()))
+        out[k] = {c: (sum(v) / len(v), len(v)) for c, v in best.items()}
+    return out
+
+
+def cmd_counters(args):
+    data = read_counters(args.csv)
+    names = sorted({c for v in data.values() for c in v})
+    print("# rocprofv3 counters per dispatch\n")
+    print("Commands:\n\n    %s\n" % args.command.replace(" && ", "\n    "))
+    print("Each kernel in its most frequent launch shape (the tuned plan's; the tuner's "
+          "trial tiles are left out).  Derived metrics are rocprofv3's own: VALUBusy and "
+          "LdsUtil in % of the CU cycles, LDSBankConflict in % of the CU cycles spent in "
+          "bank conflicts, OccupancyPercent in % of the wave slots.\n")
+    print("| kernel | dispatches | " + " | ".join(names) +
+          " | VALU insts / wave | LDS insts / wave |")
+    print("|---|---:|" + "---:|" * (len(names) + 2))
+    keep = [k for k in data if base_name(k) in STAGE_OF]
+    keep.sort(key=lambda k: (STAGE_OF[base_name(k)], short_name(k)))
+    for k in keep:
+        v = data[k]
+        n = max(c[1] for c in v.values())
+        get = lambda c: v[c][0] if c in v else float("nan")
+        div = lambda a, b: a / b if b else float("nan")
+        waves = get("SQ_WAVES")
+        extra = [div(get("SQ_INSTS_VALU"), waves), div(get("SQ_INSTS_LDS"), waves)]
+        cells = ["%.4g" % get(c) for c in names] + ["%.1f" % x for x in extra]
+        print("| `%s` | %d | %s |" % (short_name(k), n, " | ".join(cells)))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -192,8 +240,11 @@ def main(argv=None):
     b.add_argument("--copy-bytes", type=int, required=True)
     b.add_argument("--json")
     b.add_argument("--command", default="")
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--command", default="")
     args = ap.parse_args(argv)
-    cmd_stats(args) if args.cmd == "stats" else cmd_traffic(args)
+    {"stats": cmd_stats, "traffic": cmd_traffic, "counters": cmd_counters}[args.cmd](args)
 
 
 if __name__ == "__main__":
