@@ -275,16 +275,17 @@ int upload_twiddles(sqc::Plan &p, int axis) {
   return upload((double **)&p.d_tw[axis], buf.data(), buf.size() * sizeof(double), p.bytes);
 }
 
-double amplitude(const sqc::Geo &g, int sp, double q) {
-  if (g.weight_kind == SQC_HIP_WEIGHT_XRAY) {
-    const double *c = g.xcoef[sp];
+double amplitude(const sqc::Plan &p, int sp, double q) {
+  const double *w = p.wtab.data() + (size_t)sp * sqc::kWtab;
+  if (p.geo.weight_kind == SQC_HIP_WEIGHT_XRAY) {
+    const double *c = w + 1;
     double st = q / (4.0 * std::acos(-1.0));
     st = st * st;
     double f = c[8];
     for (int i = 0; i < 4; ++i) f = f + c[i] * std::exp(-c[4 + i] * st);
     return f;
   }
-  return g.amp[sp];
+  return w[0];
 }
 
 // Quanta of the exact shell sums (the x pass's exact_parts).  A value added
@@ -310,14 +311,15 @@ void set_shell_quanta(sqc::Plan &p, double qmax) {
   double A = 0.0, nn = 0.0;
   for (int a = 0; a < S; ++a) {
     double w;
+    const double *wa = p.wtab.data() + (size_t)a * sqc::kWtab;
     if (g.weight_kind == SQC_HIP_WEIGHT_XRAY) {
       // |f_a(q)| <= sum_i |a_i| max(1, exp(-b_i s^2)) + |c| over the q range
-      const double *c = g.xcoef[a];
+      const double *c = wa + 1;
       w = std::fabs(c[8]);
       for (int i = 0; i < 4; ++i)
         w += std::fabs(c[i]) * std::max(1.0, std::exp(-c[4 + i] * smax * smax));
     } else {
-      w = std::fabs(g.amp[a]);
+      w = std::fabs(wa[0]);
     }
     A += w * (double)p.counts[a];
     for (int b = a; b < S; ++b) nn = std::max(nn, (double)p.counts[a] * (double)p.counts[b]);
@@ -446,6 +448,15 @@ int alloc_bins(sqc::Plan &p) {
   return 0;
 }
 
+// Shell sums kept in HBM rows (many species with partials) are at most
+// 512 MB together: fewer persistent workgroups beyond that.
+void cap_rows(sqc::Plan &p) {
+  if (!p.geo.acc_global) return;
+  const size_t row = (size_t)p.geo.xwords * p.geo.ncomp * p.geo.nq * sizeof(uint64_t);
+  const int cap = (int)std::max<size_t>(1, ((size_t)512 << 20) / row);
+  p.nblk3 = std::min(p.nblk3, std::max(cap, std::min(p.ncu, p.nblk3)));
+}
+
 // x pass: lb lines x F fields of n1 points, the twiddles, the shell
 // accumulators and the 1 / phihat_1 table.
 size_t xpass_lds(const sqc::Plan &p, int lb) {
@@ -466,6 +477,7 @@ void set_xpass(sqc::Plan &p, int lb) {
       1, std::min<size_t>(8, kLdsCap / (p.lds3 + sqc::static_lds_xpass())));
   const int ngroups = (p.geo.nlines + lb - 1) / lb;
   p.nblk3 = std::max(1, std::min(ngroups, resident * p.ncu));
+  cap_rows(p);
 }
 
 // Per-workgroup shell rows of the x pass (one row per persistent workgroup).
@@ -493,7 +505,8 @@ void free_plan(sqc_hip_plan *h) {
   void *bufs[] = {p.d_pos, p.d_i0, p.d_i0s, p.d_phi, p.d_dphi, p.d_key, p.d_count, p.d_start, p.d_cursor, p.d_sidx,
                   p.d_species, p.d_W1, p.d_W1b, p.d_W2, p.d_tw[0], p.d_tw[1], p.d_tw[2], p.d_ph[0],
                   p.d_ph[1], p.d_ph[2], p.d_lo2, p.d_hi2, p.d_loff, p.d_line_info,
-                  p.d_line_off, p.d_line_d23, p.d_stab, p.d_den, p.d_cnt, p.d_wgpart, p.d_acc, p.d_frame,
+                  p.d_line_off, p.d_line_d23, p.d_stab, p.d_den, p.d_cnt, p.d_wtab, p.d_wgpart, p.d_acc,
+                  p.d_frame,
                   p.d_gnum, p.d_gpart, p.d_slots, p.d_gs, p.d_rho, h->g_pos, h->g_out, h->g_gs,
                   h->g_grad, h->g_rho};
   for (void *b : bufs)
@@ -524,11 +537,17 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   g.nq = cfg->nq;
   g.weight_kind = cfg->weight_kind;
   g.partials = cfg->partials ? 1 : 0;
+  p.wtab.assign((size_t)S * sqc::kWtab, 0.0);
   for (int a = 0; a < S; ++a) {
     // strengths: 1 under unit and X-ray weights (the X-ray amplitude is
     // f_a(|q|), applied in the x-pass epilogue), b_a under neutron weights
-    g.amp[a] = cfg->weight_kind == SQC_HIP_WEIGHT_NEUTRON ? cfg->amp_const[a] : 1.0;
-    for (int i = 0; i < 9; ++i) g.xcoef[a][i] = cfg->xray_coef ? cfg->xray_coef[9 * a + i] : 0.0;
+    double *w = p.wtab.data() + (size_t)a * sqc::kWtab;
+    w[0] = cfg->weight_kind == SQC_HIP_WEIGHT_NEUTRON ? cfg->amp_const[a] : 1.0;
+    for (int i = 0; i < 9; ++i) w[1 + i] = cfg->xray_coef ? cfg->xray_coef[9 * a + i] : 0.0;
+    if (a < sqc::kGeoSpecies) {
+      g.amp[a] = w[0];
+      for (int i = 0; i < 9; ++i) g.xcoef[a][i] = w[1 + i];
+    }
   }
   p.species_of.assign(cfg->species_of, cfg->species_of + cfg->natoms);
   p.counts.assign(S, 0);
@@ -546,7 +565,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   // as the strength; its pair column is of the unweighted amplitude
   g.pair_scale = 1.0;
   if (g.F == 1 && cfg->weight_kind != SQC_HIP_WEIGHT_XRAY)
-    g.pair_scale = g.amp[0] != 0.0 ? 1.0 / (g.amp[0] * g.amp[0]) : 0.0;
+    g.pair_scale = p.wtab[0] != 0.0 ? 1.0 / (p.wtab[0] * p.wtab[0]) : 0.0;
 
   // --- cell and modes ---
   const double two_pi = 2.0 * std::acos(-1.0);
@@ -595,7 +614,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
           double sw = 0.0, sw2 = 0.0;
           for (int t = 0; t < S; ++t) {
             if (p.counts[t] <= 0) continue;
-            const double w = amplitude(g, t, ql);
+            const double w = amplitude(p, t, ql);
             sw = sw + (double)p.counts[t] * w;
             sw2 = sw2 + (double)p.counts[t] * w * w;
           }
@@ -823,6 +842,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if ((rc = upload(&p.d_line_off, p.line_off.data(), p.line_off.size() * 4, p.bytes))) return rc;
   if ((rc = upload(&p.d_stab, stab.data(), stab.size() * 2, p.bytes))) return rc;
   if ((rc = upload(&p.d_den, p.den.data(), p.den.size() * 8, p.bytes))) return rc;
+  if ((rc = upload(&p.d_wtab, p.wtab.data(), p.wtab.size() * 8, p.bytes))) return rc;
   {
     std::vector<double> cnt(p.shell_count.begin(), p.shell_count.end());
     if ((rc = upload(&p.d_cnt, cnt.data(), cnt.size() * 8, p.bytes))) return rc;
@@ -1092,6 +1112,7 @@ int run_frames(sqc::Plan &p, const double *pos_dev, int64_t nframes) {
 }  // namespace
 
 namespace sqc {
+void cap_wgpart_rows(Plan &p) { cap_rows(p); }
 // The thread's error message for sqc_hip_last_error, set by the other
 // translation units of the library.
 int set_error(int code, const std::string &msg) { return fail(code, msg); }

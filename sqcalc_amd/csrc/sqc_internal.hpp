@@ -23,7 +23,13 @@
 namespace sqc {
 
 constexpr int kMaxStages = 12;
-constexpr int kMaxSpecies = 8;
+// Species a plan takes, and the species whose weights ride in Geo itself
+// (plans with more read them from the plan's weight table in HBM, and the
+// x passes run their any-field-count instance).
+constexpr int kMaxSpecies = 32;
+constexpr int kGeoSpecies = 8;
+// Weight table: per species its constant amplitude and IT92 a1..a4 b1..b4 c.
+constexpr int kWtab = 10;
 constexpr uint16_t kNoShell = 0xFFFF;
 // timed stages: 0 atom sort, 1 spread+z, 2 y, 3 x+bin, 4 finalize,
 // 5 adjoint x, 6 adjoint y, 7 adjoint z+gather, 8 adjoint slot sum
@@ -72,7 +78,7 @@ struct Geo {
   int nlines;
   int partials;
   int64_t natoms;
-  double amp[kMaxSpecies];
+  double amp[kGeoSpecies];
   // factor on the pair sums: 1, or 1 / amp[0]^2 for a one-field plan under
   // neutron or unit weights, whose single field carries the species weight
   // in its strength (the pair columns are of the unweighted amplitudes)
@@ -85,7 +91,7 @@ struct Geo {
   double xq_inv[2];
   double xq_lo;
   int xwords;
-  double xcoef[kMaxSpecies][9];
+  double xcoef[kGeoSpecies][9];
   int nxcd;         // XCDs the workgroups are dealt to
 };
 
@@ -104,6 +110,7 @@ struct Plan {
   int64_t ncand = 0;          // candidate slots in the line table
   std::vector<int32_t> species_of;
   std::vector<int64_t> counts;  // atoms per species
+  std::vector<double> wtab;     // [S][kWtab] weight table (host copy of d_wtab)
   std::vector<double> den;
   std::vector<int64_t> shell_count;
   // per h3 = 0..M3: first line and the h2 interval
@@ -148,6 +155,7 @@ struct Plan {
   uint16_t *d_stab = nullptr;
   double *d_line_d23 = nullptr;  // per x line: 1 / (phihat_2 phihat_3)
   double *d_den = nullptr, *d_cnt = nullptr;
+  double *d_wtab = nullptr;      // [S][kWtab]
   unsigned long long *d_wgpart = nullptr;
   double *d_acc = nullptr, *d_frame = nullptr;
   double *d_gnum = nullptr;
@@ -187,6 +195,9 @@ int kernel_lds_limits(Plan &p);
 size_t static_lds_spread();
 size_t static_lds_xpass();
 void persistent_grids(Plan &p);
+// Fewer persistent x-pass workgroups when their HBM shell rows would exceed
+// 512 MB (sqc_host.cpp).
+void cap_wgpart_rows(Plan &p);
 int collect_stage_times(Plan &p);
 
 }  // namespace sqc

@@ -835,7 +835,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
                                                  const int32_t *__restrict__ start,
                                                  const int32_t *__restrict__ species,
                                                  const typename C2<T>::t *__restrict__ twz,
-                                                 typename C2<T>::t *__restrict__ W1) {
+                                                 typename C2<T>::t *__restrict__ W1,
+                                                 const double *__restrict__ wtab) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Stage st;
@@ -877,7 +878,11 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     const int a = r.w & 0xFFFF;
     // lane l < ox*oy holds the xy weight and the LDS offset of xy cell l
     // (ox <= txr and oy <= ty with txr * ty <= 32, so ox*oy <= 64)
-    const T w = (g.F == 1) ? (T)g.amp[(r.w >> 16) & 0xFF] : T(1);
+    T w = T(1);
+    if (g.F == 1) {
+      const int sp = (r.w >> 16) & 0xFF;
+      w = g.S > kGeoSpecies ? (T)wtab[sp * kWtab] : (T)g.amp[sp];
+    }
     const int cells = ox * oy * ns;
     if (a >= zs0 && a + ns <= zs1 && cells <= 64 * kWholeRounds) {
       spread_whole<T>(tflat, ph, w, ox, kxl, xll, kyl, yll, ns, ccx, cs, cells, a, po, wm, lane);
@@ -1126,8 +1131,11 @@ __device__ __forceinline__ Parts exact_parts(double v, double inv_q, double lo_s
 }
 
 // FC: the number of fields when the epilogue is compiled for it (0: g.F at
-// run time).  With more than one field, fields and species coincide.  LB:
-// the lines per group when compiled in (2, the common tuned value; 0: lb_).
+// run time, at most kGeoSpecies, with the species amplitudes of a mode in
+// registers; -1: any g.F, the amplitudes re-read from the LDS rows and the
+// weights from the plan's table).  With more than one field, fields and
+// species coincide.  LB: the lines per group when compiled in (2, the common
+// tuned value; 0: lb_).
 template <typename T, int N, int FC, int LB>
 __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, int ngroups,
                                                   const typename C2<T>::t *__restrict__ W2,
@@ -1140,13 +1148,14 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
                                                   unsigned long long *__restrict__ wgpart,
                                                   typename C2<T>::t *__restrict__ rho_out,
                                                   int64_t ncand, double *__restrict__ gnum,
-                                                  double *__restrict__ gpart) {
+                                                  double *__restrict__ gpart,
+                                                  const double *__restrict__ wtab) {
   using V = typename C2<T>::t;
   const int lb = LB > 0 ? LB : lb_;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LineGroup lg;
   const int n1 = g.n[0], F = FC > 0 ? FC : g.F, nq = g.nq, ncomp = g.ncomp, M1 = g.M[0];
-  constexpr int FM = FC > 0 ? FC : kMaxSpecies;
+  constexpr int FM = FC > 0 ? FC : kGeoSpecies;
   const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
   V *tw = rows + (size_t)lb * F * rs;
@@ -1158,7 +1167,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
   // LDS instructions)
   // only the run-time field-count instance keeps sums in HBM (the host
   // routes every such plan there), so the compiled instances keep pure LDS adds
-  const bool gacc = FC == 0 && g.acc_global != 0;
+  const bool gacc = FC <= 0 && g.acc_global != 0;
   const int ntot = ncomp * nq, nw = g.xwords * ntot;
   unsigned long long *acc = reinterpret_cast<unsigned long long *>(tw + n1);
   unsigned long long *accg = wgpart + (size_t)blockIdx.x * nw;
@@ -1227,7 +1236,47 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
     // candidate c of line li, kept in shell s
     // the factor on the pair sums (1 whenever there are two or more fields)
     const double ps = FC == 2 ? 1.0 : g.pair_scale;
+    // any number of fields: the amplitudes are read from the rows again for
+    // the pair sums, and the weights come from the plan's table
+    auto bin_mode_any = [&](int c, int li, unsigned s) {
+      const int h2 = lg.h[li][0], h3 = lg.h[li][1];
+      const int h1 = lg.h[li][2] + (c - lg.off[li]);
+      const double d = lg.d23[li] * inv1[abs(h1)];
+      const int ix = h1 < 0 ? h1 + n1 : h1;
+      const V *rp = rows + (size_t)li * F * rs + ix;
+      const bool xr = g.weight_kind == 2;
+      const double q = xr ? qlen_of(g, h1, h2, h3) : 0.0;
+      double tr = 0.0, ti = 0.0;
+      for (int a = 0; a < F; ++a) {
+        const V v = rp[(size_t)a * rs];
+        const double ar = (double)v.x * d, ai = (double)v.y * d;
+        if (rho_out) rho_out[(size_t)a * ncand + c] = V{(T)ar, (T)ai};
+        const double w = xr ? xray_f(wtab + (size_t)a * kWtab + 1, q) : wtab[(size_t)a * kWtab];
+        tr += w * ar;
+        ti += w * ai;
+      }
+      const double val = tr * tr + ti * ti;
+      add(s, 2.0 * val);
+      if (gnum) gnum[c] += val;
+      if (g.partials || gpart) {
+        int pidx = 0;
+        for (int a = 0; a < F; ++a) {
+          const V va = rp[(size_t)a * rs];
+          const double ar = (double)va.x * d, ai = (double)va.y * d;
+          for (int b = a; b < F; ++b, ++pidx) {
+            const V vb = rp[(size_t)b * rs];
+            const double pv = g.pair_scale * (ar * ((double)vb.x * d) + ai * ((double)vb.y * d));
+            if (g.partials) add((pidx + 1) * nq + s, 2.0 * pv);
+            if (gpart) gpart[(size_t)pidx * ncand + c] += pv;
+          }
+        }
+      }
+    };
     auto bin_mode = [&](int c, int li, unsigned s) {
+      if constexpr (FC < 0) {
+        bin_mode_any(c, li, s);
+        return;
+      }
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
       const int h1 = lg.h[li][2] + (c - lg.off[li]);
       const double d = lg.d23[li] * inv1[abs(h1)];
@@ -1381,13 +1430,14 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
                                               const double *__restrict__ cnt,
                                               const double *__restrict__ gS,
                                               const double *__restrict__ gP,
-                                              typename C2<T>::t *__restrict__ W2) {
+                                              typename C2<T>::t *__restrict__ W2,
+                                              const double *__restrict__ wtab) {
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   // fields, species and adjoint fields coincide when there is more than one
   const int n1 = g.n[0], F = FC > 0 ? FC : g.F, nq = g.nq, Fa = FC > 0 ? FC : Fa_;
   const int S = F > 1 ? F : g.S;
-  constexpr int FM = FC > 0 ? FC : kMaxSpecies;
+  constexpr int FM = FC > 0 ? FC : kGeoSpecies;
   __shared__ LineGroup lg;
   const int rs = g.rs1;
   V *rows = reinterpret_cast<V *>(smem);
@@ -1428,7 +1478,49 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
     // candidate c of line li, kept in shell s
     // the factor on the pair sums (1 whenever there are two or more fields)
     const double ps = FC == 2 ? 1.0 : g.pair_scale;
+    // any number of fields (F = Fa > kGeoSpecies): the amplitudes are read
+    // from rho per use and the weights from the plan's table
+    auto amplify_any = [&](int c, int li, unsigned s) {
+      const int h2 = lg.h[li][0], h3 = lg.h[li][1];
+      const int h1 = lg.h[li][2] + (c - lg.off[li]);
+      const double d = lg.d23[li] * inv1[abs(h1)];
+      const int ix = h1 < 0 ? h1 + n1 : h1;
+      const bool xr = g.weight_kind == 2;
+      const double q = xr ? qlen_of(g, h1, h2, h3) : 0.0;
+      auto wt = [&](int a) {
+        return xr ? xray_f(wtab + (size_t)a * kWtab + 1, q) : wtab[(size_t)a * kWtab];
+      };
+      double tr = 0.0, ti = 0.0;
+      for (int a = 0; a < F; ++a) {
+        const V v = rho[(size_t)a * ncand + c];
+        const double w = wt(a);
+        tr += w * (double)v.x;
+        ti += w * (double)v.y;
+      }
+      const double G = Gsh[s];
+      const double ic = gP ? inc[s] : 0.0;
+      for (int cf = 0; cf < Fa; ++cf) {
+        const double wc = wt(cf);
+        double cr = 2.0 * G * wc * tr, ci = -2.0 * G * wc * ti;
+        if (gP) {
+          // the pairs (a, b) with a = cf or b = cf
+          for (int o = 0; o < F; ++o) {
+            const int a = o < cf ? o : cf, b = o < cf ? cf : o;
+            const V vo = rho[(size_t)o * ncand + c];
+            double H = gP[((size_t)a * S + b) * nq + s] * ic * g.pair_scale;
+            if (o == cf) H *= 2.0;  // the pair (cf, cf) counts both factors
+            cr += H * (double)vo.x;
+            ci -= H * (double)vo.y;
+          }
+        }
+        rows[((size_t)li * Fa + cf) * rs + (ix)] = V{(T)(cr * d), (T)(ci * d)};
+      }
+    };
     auto amplify = [&](int c, int li, unsigned s, V r0) {
+      if constexpr (FC < 0) {
+        amplify_any(c, li, s);
+        return;
+      }
       const int h2 = lg.h[li][0], h3 = lg.h[li][1];
       const int h1 = lg.h[li][2] + (c - lg.off[li]);
       const double d = lg.d23[li] * inv1[abs(h1)];
@@ -1740,7 +1832,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
 // dL/dr = sum_i b_i (n_i / 2 pi) dV/dX_i, dV/dX_i = -sum_g V phi'(u_i) prod phi.
 __global__ void k_adj_sum(Geo g, const int4 *__restrict__ i0a,
                           const int32_t *__restrict__ species, const double *__restrict__ slots,
-                          int Fa, double *__restrict__ grad) {
+                          int Fa, double *__restrict__ grad, const double *__restrict__ wtab) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= g.natoms) return;
   const int4 i0 = i0a[j];
@@ -1765,7 +1857,8 @@ __global__ void k_adj_sum(Geo g, const int4 *__restrict__ i0a,
       s2 += o[2];
     }
   }
-  const double w = (Fa == 1) ? g.amp[species[j]] : 1.0;
+  double w = 1.0;
+  if (Fa == 1) w = g.S > kGeoSpecies ? wtab[species[j] * kWtab] : g.amp[species[j]];
   const double inv_two_pi = 1.0 / (2.0 * 3.14159265358979323846);
   const double d0 = -s0 * g.n[0] * inv_two_pi * w;
   const double d1 = -s1 * g.n[1] * inv_two_pi * w;
@@ -1874,6 +1967,7 @@ template <typename Fn> static void pick_chunk(int tx2, Fn &&fn) {
 // species with its pair column take the run-time instance.
 template <typename Fn> static void pick_fields(const Geo &g, Fn &&fn) {
   const int F = g.F;
+  if (F > kGeoSpecies) return fn(std::integral_constant<int, -1>{});
   if (g.acc_global) return fn(std::integral_constant<int, 0>{});  // shell sums in HBM
   if (F == 1 && g.weight_kind != 2 && !g.partials) return fn(std::integral_constant<int, 1>{});
   if (F == 2) return fn(std::integral_constant<int, 2>{});
@@ -1915,7 +2009,7 @@ static int forward_front_t(Plan &p, const double *d_pos, void *W1, hipStream_t s
         hipLaunchKernelGGL((k_spread_z<T, decltype(n)::value, decltype(tt)::value>),
                            dim3(g.ntx * g.nty, g.F), dim3(NT), p.lds1, st, g, p.ax[2], p.d_i0s,
                            (const T *)p.d_phi, p.d_sidx, p.d_start, p.d_species,
-                           (const V *)p.d_tw[2], (V *)W1);
+                           (const V *)p.d_tw[2], (V *)W1, p.d_wtab);
       };
       if (g.txr == 8 && g.ty == 2)
         go(std::integral_constant<int, 1>{});
@@ -1962,7 +2056,7 @@ static int forward_x_t(Plan &p, const void *W2, hipStream_t st, int mode, double
               dim3(p.nblk3), dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups, (const V *)W2,
               (const V *)p.d_tw[0], (const int4 *)p.d_line_info, p.d_line_off, p.d_stab,
               p.d_ph[0], p.d_line_d23, p.d_wgpart, (V *)rho_out, (int64_t)p.ncand,
-              mode == 0 ? p.d_gnum : nullptr, mode == 0 ? p.d_gpart : nullptr);
+              mode == 0 ? p.d_gnum : nullptr, mode == 0 ? p.d_gpart : nullptr, p.d_wtab);
         };
         if (p.lb == 2)
           go(std::integral_constant<int, 2>{});
@@ -2027,7 +2121,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
                            dim3(NT), p.lds3, st, g, p.ax[0], p.lb, ngroups, Fa, (const V *)rho,
                            (int64_t)p.ncand, (const V *)p.d_tw[0], (const int4 *)p.d_line_info,
                            p.d_line_off, p.d_stab, p.d_ph[0], p.d_line_d23, p.d_den, p.d_cnt,
-                           gS, gP, (V *)p.d_W2);
+                           gS, gP, (V *)p.d_W2, p.d_wtab);
       });
     });
   }
@@ -2061,7 +2155,7 @@ static int adjoint_t(Plan &p, const double *d_pos, const void *rho, const double
   {
     StageTimer t(p, 8);
     hipLaunchKernelGGL(k_adj_sum, dim3(ga), dim3(blk), 0, st, g, p.d_i0, p.d_species, slots,
-                       Fa, grad);
+                       Fa, grad, p.d_wtab);
   }
   SQC_CHECK(hipGetLastError());
   return 0;
@@ -2140,6 +2234,7 @@ static void persistent_grids_t(Plan &p) {
   if (nx > 0) {
     const int ngroups = (g.nlines + p.lb - 1) / p.lb;
     p.nblk3 = std::max(1, std::min(ngroups, nx * p.ncu));
+    cap_wgpart_rows(p);
   }
 }
 

@@ -227,8 +227,8 @@ contains
          present(frame%type_id(j)) = .true.
       end do
       nspec = count(present)
-      if (nspec > 8) then
-         write (error_unit, '(a)') 'sqcalc_hip: at most 8 atom types are supported'
+      if (nspec > 32) then
+         write (error_unit, '(a)') 'sqcalc_hip: at most 32 atom types are supported'
          stop 6
       end if
       allocate (spec_type(nspec), counts(nspec), spec_label(nspec))

@@ -17,16 +17,23 @@ pytestmark = pytest.mark.gpu
 TOL = {"double": 1e-8, "single": 1e-5}
 
 
-MAPPINGS = {2: {1: "Si", 2: "O"}, 4: {1: "Si", 2: "O", 3: "Na", 4: "Ca"}}
+MAPPINGS = {2: {1: "Si", 2: "O"}, 4: {1: "Si", 2: "O", 3: "Na", 4: "Ca"},
+            10: {1: "Si", 2: "O", 3: "Na", 4: "Ca", 5: "Al", 6: "Mg", 7: "Fe", 8: "K", 9: "C",
+                 10: "N"}}
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
 @pytest.mark.parametrize("weight,partials,nspecies", [("neutron", False, 2), ("unit", True, 2),
                                                       ("xray", True, 2), ("xray", False, 2),
-                                                      ("xray", True, 4), ("neutron", True, 4)])
+                                                      ("xray", True, 4), ("neutron", True, 4),
+                                                      ("neutron", False, 10), ("xray", True, 10),
+                                                      ("neutron", True, 10)])
 def test_adjoint_matches_oracle(hip_device, precision, weight, partials, nspecies):
     """Four species run the x pass and its adjoint with the field count read
-    at run time (the compiled instances hold one and two fields)."""
+    at run time (the compiled instances hold one and two fields); ten run the
+    any-field-count instances, with the weights read from the plan's table
+    (and, for one weighted field, the spreader's and the slot sum's
+    per-species strengths as well)."""
     import torch
 
     from sqcalc_amd import HipStructureFactor

@@ -155,6 +155,54 @@ def test_eight_species_shell_sums(hip_device, precision, nq, in_hbm):
     assert max(devs) <= TOL[precision]
 
 
+TEN = {1: "Si", 2: "O", 3: "Na", 4: "Ca", 5: "Al", 6: "Mg", 7: "Fe", 8: "K", 9: "C", 10: "N"}
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("weight,nq,in_hbm", [("unit", 40, 0), ("xray", 40, 0), ("neutron", 600, 1)])
+def test_ten_species_against_oracle(hip_device, precision, weight, nq, in_hbm):
+    """Ten species with partials: the any-field-count x pass (weights from the
+    plan's table, the amplitudes re-read for 55 pair columns), with the shell
+    sums in LDS at 40 shells and in HBM rows at 600.  Every column against
+    the oracle."""
+    cell, origin, types, frames = random_case(natoms=600, length=13.0, nframes=2, seed=29,
+                                              ntypes=10, tilt=(0.4, -0.2, 0.3))
+    print("case ten species", precision, weight, nq, flush=True)
+    qmax = 3.0
+    sf = _hip(cell, origin, types, TEN, weight, "mean", 0.0, qmax, nq, precision,
+              eps=1e-10 if precision == "double" else 1e-6)
+    assert sf.info.nfields == 10
+    assert sf.info.shell_sums_in_hbm == in_hbm
+    sf.accumulate(frames)
+    table = sf.table()
+    sf.close()
+    assert table.shape[1] == 2 + 55
+    ref = oracle_run(cell, origin, types, TEN, weight, "mean", 0.0, qmax, nq, frames).table()
+    devs = [rel_dev(ref[:, c], table[:, c]) for c in range(1, ref.shape[1])]
+    print("  worst column %.2e" % max(devs))
+    assert max(devs) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_ten_species_one_weighted_field(hip_device, precision):
+    """Ten species, neutron weights, no partials: one field whose atoms
+    carry their species' scattering lengths, read from the plan's table."""
+    cell, origin, types, frames = random_case(natoms=600, length=13.0, nframes=2, seed=31,
+                                              ntypes=10)
+    print("case ten species one field", precision, flush=True)
+    sf = _hip(cell, origin, types, TEN, "neutron", "mean", 0.0, 4.0, 50, precision,
+              partials=False, eps=1e-10 if precision == "double" else 1e-6)
+    assert sf.info.nfields == 1
+    sf.accumulate(frames)
+    got = sf.table()
+    sf.close()
+    ref = oracle_run(cell, origin, types, TEN, "neutron", "mean", 0.0, 4.0, 50, frames,
+                     partials=False).table()
+    dev = rel_dev(ref[:, 1], got[:, 1])
+    print("  max deviation %.2e" % dev)
+    assert dev <= TOL[precision]
+
+
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_faber_ziman_against_oracle_and_high_q_limits(hip_device, precision):
     """Faber-Ziman columns A_ab = (S_ab - x_a d_ab) / (x_a x_b) + 1 against the
