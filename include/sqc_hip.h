@@ -75,7 +75,7 @@ typedef struct sqc_hip_config {
   double cell[9];           /* lattice vectors in columns */
   double origin[3];         /* lower corner of the cell */
   int64_t natoms;
-  int32_t nspecies;
+  int32_t nspecies;         /* species present, 1..32 */
   int32_t mode_partials;    /* with partials: also accumulate Re(rho_a rho_b*) per
                                kept mode (the pair sums behind the --xrd pair
                                columns, src/sqc_structure_factor.f90:538-588) */
