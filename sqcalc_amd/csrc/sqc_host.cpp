@@ -351,19 +351,29 @@ size_t lds_pref_y(const sqc::Plan &p) { return p.prec == 1 ? 40 * 1024 : 64 * 10
 
 size_t vsize(const sqc::Plan &p) { return p.prec == 1 ? 8 : 16; }
 
+// Scratch lines of the long-line DFT: one line per wave for axes longer
+// than the in-place DFT takes.
+size_t long_scratch(const sqc::Plan &p, int n) {
+  return n > sqc::kMaxLine ? (size_t)(sqc::kThreads / 64) * n * vsize(p) : 0;
+}
+
 size_t spread_lds(const sqc::Plan &p, int txr, int ty) {
   const sqc::Geo &g = p.geo;
-  // the tile and the staged atom records; the DFT twiddles are read from
-  // global memory (L1-resident), which leaves room for more workgroups
-  return (size_t)(txr / 2) * ty * g.cs * vsize(p) + sqc::kStageCap * 16;
+  // the tile and the staged atom records (or, during the DFT, the long-line
+  // scratch in their place); the DFT twiddles are read from global memory
+  // (L1-resident), which leaves room for more workgroups
+  return (size_t)(txr / 2) * ty * g.cs * vsize(p) +
+         std::max<size_t>(sqc::kStageCap * 16, long_scratch(p, g.n[2]));
 }
 
 // The gather adds a per-wave scratch of the kernel values of the atoms of
 // one round: (64 / lanes per atom) atoms x 6 ns values.
 size_t gather_lds(const sqc::Plan &p, int txr, int ty) {
+  const sqc::Geo &g = p.geo;
   const int lpa = std::max(8, txr * ty);
-  return spread_lds(p, txr, ty) +
-         (size_t)(sqc::kThreads / 64) * (64 / lpa) * 6 * p.geo.ns * (vsize(p) / 2);
+  const size_t after = sqc::kStageCap * 16 +
+                       (size_t)(sqc::kThreads / 64) * (64 / lpa) * 6 * g.ns * (vsize(p) / 2);
+  return (size_t)(txr / 2) * ty * g.cs * vsize(p) + std::max(after, long_scratch(p, g.n[2]));
 }
 
 // LDS a CU offers the dynamic part of an x-pass workgroup.
@@ -381,7 +391,7 @@ int ystride(const sqc::Plan &p, int tx2) {
 }
 
 size_t ypass_lds(const sqc::Plan &p, int tx2) {
-  return (size_t)tx2 * ystride(p, tx2) * vsize(p);
+  return (size_t)tx2 * ystride(p, tx2) * vsize(p) + long_scratch(p, p.geo.n[1]);
 }
 
 int ilog2i(int v) {
@@ -465,7 +475,8 @@ size_t xpass_lds(const sqc::Plan &p, int lb) {
   // HBM; the adjoint x pass keeps two rows of nq there
   const size_t rows = g.acc_global ? 2 : std::max<size_t>(2, (size_t)g.xwords * g.ncomp);
   const size_t accb = (rows * g.nq + g.M[0] + 1) * sizeof(double);
-  return ((size_t)lb * g.F * g.rs1 + g.n[0]) * vsize(p) + accb;
+  const size_t ls = long_scratch(p, g.n[0]);
+  return ((size_t)lb * g.F * g.rs1 + g.n[0]) * vsize(p) + accb + (ls ? 16 + ls : 0);
 }
 
 // Lines per group and the persistent grid: as many workgroups as the CUs
@@ -731,10 +742,10 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   const size_t lds_pref = ::lds_pref(p);
   const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
   for (int i = 0; i < 3; ++i)
-    if (g.n[i] > sqc::kMaxLine)
+    if (g.n[i] > sqc::kMaxLongLine)
       return fail(SQC_HIP_ERR_MAXNALLOC, "fine grid of " + std::to_string(g.n[i]) +
                                              " points per axis exceeds the LDS DFT limit of " +
-                                             std::to_string(sqc::kMaxLine));
+                                             std::to_string(sqc::kMaxLongLine));
   // LDS z column stride of the spread tile: 24 words mod 64 spreads the
   // neighbouring columns of a spread update over the banks
   // (line buffers hold n + n/32 + 1 points: the DFT's padded stage layout)
@@ -805,10 +816,11 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   if ((rc = alloc(&p.d_sidx, sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = upload(&p.d_species, p.species_of.data(), sizeof(int32_t) * N, p.bytes))) return rc;
   if ((rc = alloc_bins(p))) return rc;
-  // the passes index W1 and W2 with 32-bit element offsets
-  if ((double)g.F * (M3 + 1) * n2 * n1 >= 2147483647.0 ||
-      (double)g.F * std::max(g.nlines, 1) * n1 >= 2147483647.0)
-    return fail(SQC_HIP_ERR_MAXNALLOC, "transform buffers exceed 2^31 elements");
+  // the passes index W1 and W2 with 64-bit offsets past a field, a line
+  // group or a plane; the offsets within one stay below 2^31
+  if ((double)(M3 + 1) * n2 * n1 >= 2147483647.0 ||
+      (double)g.F * std::max(g.nlines, 1) >= 2147483647.0)
+    return fail(SQC_HIP_ERR_MAXNALLOC, "transform buffers exceed 2^31 elements per field");
   const size_t w1 = (size_t)g.F * (M3 + 1) * n2 * n1 * vs;
   const size_t w2 = (size_t)g.F * std::max(g.nlines, 1) * n1 * vs;
   if ((rc = alloc((char **)&p.d_W1, w1, p.bytes))) return rc;

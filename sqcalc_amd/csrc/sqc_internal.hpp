@@ -44,8 +44,11 @@ struct AxisPlan {
   uint32_t magic[kMaxStages];
 };
 
-// Longest line the wave-per-line DFT handles (fine-grid points per axis).
+// Longest line whose DFT stages run in place in registers (fine-grid points
+// per axis); longer lines, up to kMaxLongLine, run out of place through a
+// scratch line per wave.
 constexpr int kMaxLine = 1024;
+constexpr int kMaxLongLine = 2048;
 // Most lines an x-pass workgroup transforms per step.
 constexpr int kMaxLb = 8;
 // Atoms staged into LDS per round of the spreader / gather.

@@ -200,6 +200,63 @@ __device__ void wave_fft_any(typename C2<T>::t *line, const AxisPlan &ax,
   }
 }
 
+// Lines longer than kMaxLine hold too many points for the in-place stages'
+// registers.  Their stages run out of place between the
+// line and a scratch line of the wave's own, one butterfly per lane at a
+// time; the result is copied back when it ends in the scratch.
+template <typename T, int R>
+__device__ void long_stage(const typename C2<T>::t *src, typename C2<T>::t *dst, int n, int Ns,
+                           uint32_t magic, const typename C2<T>::t *tw, int lane) {
+  using V = typename C2<T>::t;
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  for (int j = lane; j < nb; j += 64) {
+    const int q = magic ? (int)__umulhi((unsigned)j, magic) : j;
+    const int k = j - q * Ns;
+    V v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+    if (Ns > 1) {
+      const int kt = k * tstep;
+      int ti = kt;
+#pragma unroll
+      for (int r = 1; r < R; ++r, ti += kt) v[r] = cmul(v[r], tw[ti]);
+    }
+    Dft<T, R>::run(v);
+    const int d0 = q * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[d0 + r * Ns] = v[r];
+  }
+  wave_sync();
+}
+
+template <typename T>
+__device__ void wave_fft_long(typename C2<T>::t *line, typename C2<T>::t *scratch,
+                              const AxisPlan &ax, const typename C2<T>::t *tw, int lane) {
+  using V = typename C2<T>::t;
+  V *a = line, *b = scratch;
+  int Ns = 1;
+  wave_sync();
+  for (int s = 0; s < ax.nst; ++s) {
+    const int R = ax.rad[s];
+    switch (R) {
+      case 2: long_stage<T, 2>(a, b, ax.n, Ns, ax.magic[s], tw, lane); break;
+      case 3: long_stage<T, 3>(a, b, ax.n, Ns, ax.magic[s], tw, lane); break;
+      case 4: long_stage<T, 4>(a, b, ax.n, Ns, ax.magic[s], tw, lane); break;
+      case 5: long_stage<T, 5>(a, b, ax.n, Ns, ax.magic[s], tw, lane); break;
+      default: long_stage<T, 8>(a, b, ax.n, Ns, ax.magic[s], tw, lane); break;
+    }
+    V *t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  if (a != line) {
+    for (int i = lane; i < ax.n; i += 64) line[i] = a[i];
+    wave_sync();
+  }
+}
+
 // Compile-time line DFT for the fine-grid sizes the benchmark workloads use:
 // the radix plan, strides and divisors are constants, so the LDS accesses of
 // a butterfly become one address plus immediate offsets.
@@ -301,15 +358,20 @@ __device__ __forceinline__ void fixed_stages(typename C2<T>::t *const *ln,
 }
 
 // Line DFT of the plan's length: compile-time when N > 0 (N == ax.n),
-// otherwise the runtime-planned stages.
+// otherwise the runtime-planned stages; lines longer than kMaxLine go
+// through `scratch` (a line of the wave's own).
 template <typename T, int N>
 __device__ __forceinline__ void wave_fft(typename C2<T>::t *line, const AxisPlan &ax,
-                                         const typename C2<T>::t *tw, int lane) {
+                                         const typename C2<T>::t *tw, int lane,
+                                         typename C2<T>::t *scratch) {
   if constexpr (N > 0) {
     typename C2<T>::t *ln[1] = {line};
     fixed_stages<T, N, 0, 1, 1>(ln, tw, lane);
   } else {
-    wave_fft_any<T>(line, ax, tw, lane);
+    if (ax.n > kMaxLine)
+      wave_fft_long<T>(line, scratch, ax, tw, lane);
+    else
+      wave_fft_any<T>(line, ax, tw, lane);
   }
 }
 
@@ -318,7 +380,8 @@ __device__ __forceinline__ void wave_fft(typename C2<T>::t *line, const AxisPlan
 template <typename T, int N>
 __device__ __forceinline__ void wave_fft_lines(typename C2<T>::t *base, int stride, int first,
                                                int nlines, const AxisPlan &ax,
-                                               const typename C2<T>::t *tw, int lane) {
+                                               const typename C2<T>::t *tw, int lane,
+                                               typename C2<T>::t *scratch) {
   int i = first;
   if constexpr (N > 0 && sizeof(T) == 4) {
     for (; i + NW < nlines; i += 2 * NW) {
@@ -326,7 +389,7 @@ __device__ __forceinline__ void wave_fft_lines(typename C2<T>::t *base, int stri
       fixed_stages<T, N, 0, 1, 2>(ln, tw, lane);
     }
   }
-  for (; i < nlines; i += NW) wave_fft<T, N>(base + (size_t)i * stride, ax, tw, lane);
+  for (; i < nlines; i += NW) wave_fft<T, N>(base + (size_t)i * stride, ax, tw, lane, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -955,7 +1018,10 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     __syncthreads();
   }
   __syncthreads();
-  wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane);
+  // long lines: the wave's scratch line lies past the tile (the staged
+  // records are not in use during the DFT)
+  wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane,
+                       tile + (size_t)cc * cs + (size_t)wave * n3);
   __syncthreads();
   // split the packed pairs and write h3 = 0..M3.  txr * ty is a power of two
   // that divides NT, so every thread keeps one (xl, yl) column and steps h3.
@@ -1013,7 +1079,8 @@ __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay,
       },
       [&](int idx, V v) { lines[(idx & tm) * ls + (idx >> lt)] = v; });
   __syncthreads();
-  wave_fft_lines<T, N>(lines, ls, wave, tx2, ay, twy, lane);
+  wave_fft_lines<T, N>(lines, ls, wave, tx2, ay, twy, lane,
+                       lines + (size_t)tx2 * ls + (size_t)wave * g.n[1]);
   __syncthreads();
   const int nl = hi - lo + 1;
   V *dst = W2 + ((size_t)f * g.nlines + loff[k3]) * n1;
@@ -1173,6 +1240,9 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
   unsigned long long *accg = wgpart + (size_t)blockIdx.x * nw;
   // 1 / phihat_1(|h1|)
   double *inv1 = reinterpret_cast<double *>(gacc ? acc : acc + nw);
+  // scratch lines of the long-line DFT, past the 1 / phihat_1 table
+  V *lscr = reinterpret_cast<V *>(
+      smem + ((reinterpret_cast<unsigned char *>(inv1 + M1 + 1) - smem + 15) & ~(ptrdiff_t)15));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   if (gacc)
     for (int i = tid; i < nw; i += NT) accg[i] = 0ull;
@@ -1215,7 +1285,7 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
         [&](int idx) {
           const int r = div_by<N>(idx, inv_n1), x = idx - r * n1;
           const int li = div_by<FC>(r, inv_f), f = r - li * F;
-          return W2[(unsigned)((f * g.nlines + l0 + li) * n1 + x)];
+          return W2[(size_t)(f * g.nlines + l0 + li) * n1 + x];
         },
         [&](int idx, V v) {
           const int r = div_by<N>(idx, inv_n1);
@@ -1231,7 +1301,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
       for (int k = 0; k < kPreShell; ++k) sv[k] = stab[min(c0 + tid + k * NT, c1 - 1)];
     }
     // one line at a time: pairs raise the register count past three waves per SIMD
-    for (int r = wave; r < nl * F; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
+    for (int r = wave; r < nl * F; r += NW)
+      wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane, lscr + (size_t)wave * n1);
     __syncthreads();
     // candidate c of line li, kept in shell s
     // the factor on the pair sums (1 whenever there are two or more fields)
@@ -1447,6 +1518,9 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
   // 1 / (N max(count, 1)), in the LDS the forward pass keeps its shell sums
   // in (two rows of nq)
   double *Gsh = inv1 + g.M[0] + 1, *inc = Gsh + nq;
+  // scratch lines of the long-line DFT, past the shell tables
+  V *lscr = reinterpret_cast<V *>(
+      smem + ((reinterpret_cast<unsigned char *>(inc + nq) - smem + 15) & ~(ptrdiff_t)15));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int i = tid; i < n1; i += NT) tw[i] = twx[i];
   for (int i = tid; i <= g.M[0]; i += NT) inv1[i] = 1.0 / ph1[i];
@@ -1587,13 +1661,14 @@ __global__ void __launch_bounds__(NT) k_adj_x(Geo g, AxisPlan ax1, int lb, int n
       if (sh != kNoShell) amplify(c, li, sh, r0);
     }
     __syncthreads();
-    for (int r = wave; r < nl * Fa; r += NW) wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane);
+    for (int r = wave; r < nl * Fa; r += NW)
+      wave_fft<T, N>(rows + (size_t)r * rs, ax1, tw, lane, lscr + (size_t)wave * n1);
     __syncthreads();
     const float inv_n1 = 1.0f / (float)n1, inv_fa = 1.0f / (float)Fa;
     for (int idx = tid; idx < nl * Fa * n1; idx += NT) {
       const int r = div_by<N>(idx, inv_n1), x = idx - r * n1;
       const int li = div_by<FC>(r, inv_fa), cf = r - li * Fa;
-      W2[(unsigned)((cf * g.nlines + l0 + li) * n1 + x)] = rows[r * rs + x];
+      W2[(size_t)(cf * g.nlines + l0 + li) * n1 + x] = rows[r * rs + x];
     }
   }
 }
@@ -1632,7 +1707,8 @@ __global__ void __launch_bounds__(NT) k_adj_y(Geo g, AxisPlan ay,
         if (x0 + (idx & tm) < n1) lines[(idx & tm) * ls + yi] = v;
       });
   __syncthreads();
-  wave_fft_lines<T, N>(lines, ls, wave, tx2, ay, twy, lane);
+  wave_fft_lines<T, N>(lines, ls, wave, tx2, ay, twy, lane,
+                       lines + (size_t)tx2 * ls + (size_t)wave * g.n[1]);
   __syncthreads();
   V *dst = W1 + ((size_t)cf * (M3 + 1) + k3) * n2 * n1;
   for (int idx = tid; idx < n2 * tx2; idx += NT) {
@@ -1694,7 +1770,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
       [&](int idx) {
         const int r = small_div(idx, inv_pw), p = idx - r * pw;
         const int k3 = small_div(r, inv_ty), yl = r - k3 * tyw;
-        const V *q = src + (unsigned)((k3 * n2 + y0 + yl) * n1 + x0 + 2 * p);
+        const V *q = src + (size_t)(k3 * n2 + y0 + yl) * n1 + x0 + 2 * p;
         return VV{q[0], q[1]};
       },
       [&](int idx, VV v) {
@@ -1706,7 +1782,10 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
         if (k3 > 0) tile[(size_t)col * cs + (n3 - k3)] = V{A.x + B.y, B.x - A.y};
       });
   __syncthreads();
-  wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane);
+  // long lines: the wave's scratch line lies past the tile (the staged
+  // records are not in use during the DFT)
+  wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane,
+                       tile + (size_t)cc * cs + (size_t)wave * n3);
   stage_window(st, g, tx, ty, cf, start);
 
   const int ns = g.ns;
