@@ -1046,13 +1046,22 @@ int tune(sqc::Plan &p, const double *pos_dev) {
 int capture_batch(sqc::Plan &p) {
   const int64_t stride = 3 * p.geo.natoms;
   const int64_t B = p.batch_cap;
+  // events: [0] the fork, then per frame S (spread done), Y (y pass done)
+  // and B (x pass done); every index is checked against the vector, so a
+  // slip in this bookkeeping is an error return, not a stray handle
   const size_t nev = 3 * (size_t)B + 1;
   while (p.gev.size() < nev) {
     hipEvent_t e = nullptr;
     HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     p.gev.push_back(e);
   }
-  hipEvent_t *evS = p.gev.data() + 1, *evY = evS + B, *evB = evY + B;
+  auto ev = [&](int kind, int64_t f) -> hipEvent_t {
+    const size_t i = kind < 0 ? 0 : 1 + (size_t)kind * (size_t)B + (size_t)f;
+    return (f >= 0 && f < B && i < p.gev.size()) ? p.gev[i] : nullptr;
+  };
+  enum { kS = 0, kY = 1, kB = 2 };
+  if (!p.stream || !p.stream2 || !p.d_W1 || !p.d_W1b)
+    return fail(SQC_HIP_ERR_BAD_ARG, "frame batch capture without its second stream and buffer");
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   int rc = 0;
@@ -1062,21 +1071,34 @@ int capture_batch(sqc::Plan &p) {
     const hipError_t e = call();
     if (e != hipSuccess) rc = (int)e;
   };
-  step([&] { return hipEventRecord(p.gev[0], p.stream); });
-  step([&] { return hipStreamWaitEvent(p.stream2, p.gev[0], 0); });
+  // an event that does not exist ends the capture with an error
+  auto record = [&](int kind, int64_t f, hipStream_t s) {
+    step([&] {
+      hipEvent_t e = ev(kind, f);
+      return e ? hipEventRecord(e, s) : hipErrorInvalidResourceHandle;
+    });
+  };
+  auto wait = [&](hipStream_t s, int kind, int64_t f) {
+    step([&] {
+      hipEvent_t e = ev(kind, f);
+      return e ? hipStreamWaitEvent(s, e, 0) : hipErrorInvalidResourceHandle;
+    });
+  };
+  step([&] { return hipEventRecord(ev(-1, 0), p.stream); });
+  step([&] { return hipStreamWaitEvent(p.stream2, ev(-1, 0), 0); });
   for (int64_t f = 0; f < B && !rc; ++f) {
     void *W1 = (f & 1) ? p.d_W1b : p.d_W1;
-    if (f >= 2) step([&] { return hipStreamWaitEvent(p.stream, evY[f - 2], 0); });  // W1 reuse
+    if (f >= 2) wait(p.stream, kY, f - 2);  // W1 reuse: the y pass of frame f - 2 has read it
     if (!rc) rc = sqc::launch_forward_front(p, p.d_pos + f * stride, W1, p.stream);
-    step([&] { return hipEventRecord(evS[f], p.stream); });
-    step([&] { return hipStreamWaitEvent(p.stream2, evS[f], 0); });
+    record(kS, f, p.stream);
+    wait(p.stream2, kS, f);
     if (!rc) rc = sqc::launch_forward_y(p, W1, p.d_W2, p.stream2);
-    step([&] { return hipEventRecord(evY[f], p.stream2); });
+    record(kY, f, p.stream2);
     if (!rc) rc = sqc::launch_forward_x(p, p.d_W2, p.stream2, 0, nullptr, nullptr);
-    step([&] { return hipEventRecord(evB[f], p.stream2); });
+    record(kB, f, p.stream2);
   }
   // join the side stream back into the capturing stream
-  step([&] { return hipStreamWaitEvent(p.stream, evB[B - 1], 0); });
+  wait(p.stream, kB, B - 1);
   const hipError_t e = hipStreamEndCapture(p.stream, &graph);
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
