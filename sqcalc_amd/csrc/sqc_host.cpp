@@ -749,8 +749,7 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   // LDS z column stride of the spread tile: 24 words mod 64 spreads the
   // neighbouring columns of a spread update over the banks
   // (line buffers hold n + n/32 + 1 points: the DFT's padded stage layout)
-  const int cap3 = n3 + n3 / 32 + 1;
-  g.cs = cap3 + ((12 - cap3) % 32 + 32) % 32;
+  g.cs = sqc::spread_col_stride(n3);
   g.rs1 = n1 + n1 / 32 + 1;
   {
     int pick = -1;

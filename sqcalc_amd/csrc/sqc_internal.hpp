@@ -55,6 +55,14 @@ constexpr int kMaxLb = 8;
 constexpr int kStageCap = 256;
 constexpr int kThreads = 256;   // threads per workgroup of the LDS kernels
 
+// LDS stride of a z column of the spread tile: room for the DFT's padded
+// stage layout (n + n/32 + 1 points), rounded up to 12 words mod 32 so that
+// the neighbouring columns of a stencil update fall on different banks.
+constexpr int spread_col_stride(int n3) {
+  const int cap = n3 + n3 / 32 + 1;
+  return cap + ((12 - cap) % 32 + 32) % 32;
+}
+
 // Everything a kernel needs to know about the geometry, passed by value.
 struct Geo {
   double ainv[9];   // A^-1, column-major

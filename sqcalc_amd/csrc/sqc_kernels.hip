@@ -903,7 +903,9 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Stage st;
-  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], cs = g.cs;
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2];
+  // the column stride is a constant for the compiled line lengths
+  const int cs = N > 0 ? spread_col_stride(N) : g.cs;
   // the tile shape: compiled in for the common 8 x 2 tile (TT = 1), read
   // at run time otherwise
   constexpr bool kt = TT == 1;
@@ -999,7 +1001,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
             const int u = (A == 2 && lane >= 32) ? ub[A * q + A - 1] : ub[A * q];
             const int kv = A == 2 ? (lane & 31) : lane;
             pb[q] = T(0);
-            if (u >= 0 && kv < 3 * ns) pb[q] = phi[(size_t)rec[b0 + u].x * 3 * ns + kv];
+            // 32-bit offsets: natoms * 3 ns < 2^31 (checked by the plan)
+            if (u >= 0 && kv < 3 * ns) pb[q] = phi[(unsigned)(rec[b0 + u].x * 3 * ns + kv)];
           }
 #pragma unroll
           for (int i = 0; i < A * kPhiRegs; ++i) {
@@ -1736,7 +1739,8 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   using V = typename C2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Stage st;
-  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2], cs = g.cs;
+  const int n1 = g.n[0], n2 = g.n[1], n3 = g.n[2], M3 = g.M[2];
+  const int cs = N > 0 ? spread_col_stride(N) : g.cs;
   // the tile shape: compiled in for the common 8 x 2 tile (TT = 1), read
   // at run time otherwise
   constexpr bool kt = TT == 1;

@@ -42,7 +42,8 @@ COPY_PATTERN = "neg"
 
 def short_name(name):
     """'void sqc::k_spread_z<float, 480>(sqc::Geo, ...)' -> 'k_spread_z<float, 480>'."""
-    n = name.split("(")[0].replace("void ", "").replace("sqc::", "").strip()
+    n = name.replace("(anonymous namespace)::", "")
+    n = n.split("(")[0].replace("void ", "").replace("sqc::", "").strip()
     return n
 
 
