@@ -1749,8 +1749,10 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   V *tile = reinterpret_cast<V *>(smem);
   int4 *rec = reinterpret_cast<int4 *>(tile + (size_t)cc * cs);
   const T *tflat = reinterpret_cast<const T *>(tile);
-  // grid (tiles, fields), as the spreader's
-  const int cf = blockIdx.y, tb = blockIdx.x;
+  // grid (tiles, fields), renumbered so that every XCD walks a contiguous
+  // range of tiles: neighbouring tiles read the two halves of the same W1
+  // cache lines, and then share one L2
+  const int cf = blockIdx.y, tb = xcd_block(blockIdx.x, gridDim.x, g.nxcd);
   const int tx = tb % g.ntx, ty = tb / g.ntx;
   const int x0 = tx * txr, y0 = ty * tty;
   const int txw = min(txr, n1 - x0), tyw = min(tty, n2 - y0);
