@@ -8,6 +8,7 @@
   and leaves S and S_ab as they are.
 * sparse type ids (:840-876): types 1 and 5 give the same result as 1 and 2;
   the shell sums are exact integer sums, so the same bits.
+* ideal gas (:193-198): <S(q)> -> 1 at large q.
 * grid table (:270-290): under unit weights and --norm self the per-mode
   table's shell averages are the 1D table (here to FP64 rounding, where the
   reference's NUFFT-vs-table check allows 0.05).
@@ -136,3 +137,20 @@ def test_grid_table_shell_averages(hip_device, precision):
     dev = rel_dev(table[:, 1], avg)
     print("  grid shell averages against the table: %.2e" % dev)
     assert dev <= (1e-12 if precision == "double" else 1e-6)
+
+
+def test_ideal_gas_high_q_limit(hip_device):
+    """Uniform random positions: <S(q)> -> 1 at large q (test/test_sqcalc.py:193-198
+    allows 0.35; 4000 atoms and 5 frames average about 7000 modes a shell)."""
+    rng = np.random.default_rng(87)
+    L, n = 30.0, 4000
+    cell = np.diag([L] * 3)
+    types = np.ones(n, dtype=np.int64)
+    frames = rng.random((5, n, 3)) * L
+    sf = _plan(cell, np.zeros(3), types, None, "unit", "self", 0.0, 6.0, 120, "single")
+    sf.accumulate(frames)
+    t = sf.table()
+    sf.close()
+    mean_s = float(np.mean(t[t[:, 0] > 3.0, 1]))
+    print("  ideal gas <S(q)> for q > 3: %.4f" % mean_s)
+    assert abs(mean_s - 1.0) < 0.02
