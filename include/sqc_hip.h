@@ -112,7 +112,7 @@ typedef struct sqc_hip_info {
   int32_t shell_sums_in_hbm; /* 1: the x pass adds its shell sums in its own row of
                                HBM (ncomp * nq doubles do not fit the LDS beside one
                                x line); 0: in LDS */
-  int32_t reserved_info;
+  int32_t x_lines;          /* x lines per workgroup group of the x pass */
 } sqc_hip_info;
 
 /* Number of visible HIP devices (cudaGetDeviceCount in src/sqc_cufinufft.f90:143-174). */

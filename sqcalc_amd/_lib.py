@@ -43,7 +43,7 @@ class Info(_c.Structure):
         ("beta", _c.c_double), ("upsampfac", _c.c_double), ("device_bytes", _c.c_int64),
         ("nlines", _c.c_int64), ("tile", _c.c_int32 * 3), ("ncomp", _c.c_int32),
         ("rho_len", _c.c_int64), ("shell_sums_in_hbm", _c.c_int32),
-        ("reserved_info", _c.c_int32),
+        ("x_lines", _c.c_int32),
     ]
 
 

@@ -1223,6 +1223,7 @@ int sqc_hip_plan_info(const sqc_hip_plan *plan, sqc_hip_info *info) {
   info->ncomp = p.geo.ncomp;
   info->rho_len = p.ncand;
   info->shell_sums_in_hbm = p.geo.acc_global;
+  info->x_lines = p.lb;
   return SQC_HIP_OK;
 }
 

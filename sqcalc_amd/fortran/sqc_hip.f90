@@ -71,7 +71,7 @@ module sqc_hip
       integer(c_int32_t) :: ncomp
       integer(c_int64_t) :: rho_len
       integer(c_int32_t) :: shell_sums_in_hbm
-      integer(c_int32_t) :: reserved_info
+      integer(c_int32_t) :: x_lines
    end type sqc_hip_info_t
 
    public :: sqc_hip_device_count, sqc_hip_plan_create, sqc_hip_plan_info, sqc_hip_shells, &
