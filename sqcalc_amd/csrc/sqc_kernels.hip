@@ -969,24 +969,33 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   wave_fft_lines<T, N>(tile, cs, wave, cc, az, twz, lane,
                        tile + (size_t)cc * cs + (size_t)wave * n3);
   __syncthreads();
-  // split the packed pairs and write h3 = 0..M3.  txr * ty is a power of two
-  // that divides NT, so every thread keeps one (xl, yl) column and steps h3.
+  // split the packed pairs and write h3 = 0..M3: a thread keeps one complex
+  // column (two real columns, adjacent in x) and steps h3, writing both real
+  // columns' values with one store.  (txr / 2) * ty is a power of two that
+  // divides NT, and txw is even (n1 and x0 are).
   const int M3 = g.M[2];
-  const int lper = ltxr + ltyy;
-  const int xl = tid & (txr - 1), yl = (tid >> ltxr) & (tty - 1);
-  if (xl >= txw || yl >= tyw) return;
+  const int lper = ltxr - 1 + ltyy;
+  const int xp = tid & (ccx - 1), yl = (tid >> (ltxr - 1)) & (tty - 1);
+  if (2 * xp >= txw || yl >= tyw) return;
   const int kstep = NT >> lper;
-  const V *colp = tile + (yl * ccx + (xl >> 1)) * cs;
+  const V *colp = tile + (yl * ccx + xp) * cs;
   const size_t plane = (size_t)n2 * n1;
-  V *out = W1 + ((size_t)f * (M3 + 1) * n2 + (y0 + yl)) * n1 + x0 + xl;
-  const T sgn = (xl & 1) ? T(-0.5) : T(0.5);
+  V *out = W1 + ((size_t)f * (M3 + 1) * n2 + (y0 + yl)) * n1 + x0 + 2 * xp;
+  const T h = T(0.5);
   for (int k3 = tid >> lper; k3 <= M3; k3 += kstep) {
     const V Z = colp[k3];
     const V Zm = colp[k3 ? n3 - k3 : 0];
-    // even column: (Z + conj Zm) / 2; odd column: (Z - conj Zm) / (2i)
-    const V o = (xl & 1) ? V{T(0.5) * (Z.y + Zm.y), sgn * (Z.x - Zm.x)}
-                         : V{sgn * (Z.x + Zm.x), sgn * (Z.y - Zm.y)};
-    out[(size_t)k3 * plane] = o;
+    // even column (Z + conj Zm) / 2, odd column (Z - conj Zm) / (2i)
+    const V e{h * (Z.x + Zm.x), h * (Z.y - Zm.y)};
+    const V o{h * (Z.y + Zm.y), h * (Zm.x - Z.x)};
+    V *dst = out + (size_t)k3 * plane;
+    if constexpr (sizeof(T) == 4) {
+      // 16-byte aligned: the element offset is even
+      *reinterpret_cast<float4 *>(dst) = float4{e.x, e.y, o.x, o.y};
+    } else {
+      dst[0] = e;
+      dst[1] = o;
+    }
   }
 }
 
