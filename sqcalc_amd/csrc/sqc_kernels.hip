@@ -617,20 +617,33 @@ struct Stage {
   int nstaged;
 };
 
-__device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
-                             const int32_t *__restrict__ start) {
+// The bins of a tile's window: thread t < nb holds bin t's first sorted
+// index and its atom count.
+struct Window {
+  int nb, s0, cnt;
+};
+__device__ __forceinline__ Window window_load(const Geo &g, int tx, int ty, int f,
+                                              const int32_t *__restrict__ start) {
   int bxs, bxn, bys, byn;
   bin_window(tx, g.hx, g.ntx, &bxs, &bxn);
   bin_window(ty, g.hy, g.nty, &bys, &byn);
-  const int nb = bxn * byn;
+  Window w{bxn * byn, 0, 0};
   const int t = threadIdx.x;
-  if (t < nb) {
+  if (t < w.nb) {
     const int by = wrapi(bys + t / bxn, g.nty);
     const int bx = wrapi(bxs + t % bxn, g.ntx);
     const int bin = (f * g.nty + by) * g.ntx + bx;
-    const int s0 = start[bin];
-    st.bin_first[t] = s0;
-    st.bin_pref[t + 1] = start[bin + 1] - s0;  // counts, scanned below
+    w.s0 = start[bin];
+    w.cnt = start[bin + 1] - w.s0;
+  }
+  return w;
+}
+__device__ void stage_window(Stage &st, const Window &w) {
+  const int nb = w.nb;
+  const int t = threadIdx.x;
+  if (t < nb) {
+    st.bin_first[t] = w.s0;
+    st.bin_pref[t + 1] = w.cnt;
   }
   __syncthreads();
   if (t < 64) {
@@ -656,6 +669,10 @@ __device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
     }
   }
   __syncthreads();
+}
+__device__ void stage_window(Stage &st, const Geo &g, int tx, int ty, int f,
+                             const int32_t *__restrict__ start) {
+  stage_window(st, window_load(g, tx, ty, f, start));
 }
 
 // Stage candidates [base, base + NT) of the window behind the ones already
@@ -865,6 +882,9 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
   const int x0 = tx * txr, y0 = ty * tty;
   const int txw = min(txr, n1 - x0), tyw = min(tty, n2 - y0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // the window's bin loads are issued before the tile clear, which they
+  // overlap
+  const Window win = window_load(g, tx, ty, f, start);
   {
     // clear the tile 16 bytes a store (cc * cs * sizeof(V) is a multiple of
     // 16: cs is even)
@@ -872,7 +892,7 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     const int n4 = cc * cs * (int)sizeof(V) / 16;
     for (int i = tid; i < n4; i += NT) t4[i] = float4{0.f, 0.f, 0.f, 0.f};
   }
-  stage_window(st, g, tx, ty, f, start);
+  stage_window(st, win);
 
   const int zs0 = (n3 * wave) / NW, zs1 = (n3 * (wave + 1)) / NW;
   const int ns = g.ns;
@@ -928,9 +948,11 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     auto walk = [&](auto per_reg) {
       constexpr int A = decltype(per_reg)::value;
       for (int b0 = 0; b0 < nst; b0 += 64) {
+        // every lane holds one record of the block; an atom's fields are
+        // read from its lane (no LDS round trip per atom)
         const int idx = b0 + lane;
-        const int wv = idx < nst ? rec[idx].w : 0;
-        unsigned long long m = __ballot((wv >> (24 + wave)) & 1);
+        const int4 mr = idx < nst ? rec[idx] : int4{0, 0, 0, 0};
+        unsigned long long m = __ballot((mr.w >> (24 + wave)) & 1);
         while (m) {
           int ub[A * kPhiRegs];
 #pragma unroll
@@ -941,16 +963,23 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
           T pb[kPhiRegs];
 #pragma unroll
           for (int q = 0; q < kPhiRegs; ++q) {
-            const int u = (A == 2 && lane >= 32) ? ub[A * q + A - 1] : ub[A * q];
+            const int u0 = ub[A * q], u1 = ub[A * q + A - 1];
+            const int j0 = u0 >= 0 ? __builtin_amdgcn_readlane(mr.x, u0) : 0;
+            const int j1 = u1 >= 0 ? __builtin_amdgcn_readlane(mr.x, u1) : 0;
+            const bool hi = A == 2 && lane >= 32;
+            const int u = hi ? u1 : u0, j = hi ? j1 : j0;
             const int kv = A == 2 ? (lane & 31) : lane;
             pb[q] = T(0);
             // 32-bit offsets: natoms * 3 ns < 2^31 (checked by the plan)
-            if (u >= 0 && kv < 3 * ns) pb[q] = phi[(unsigned)(rec[b0 + u].x * 3 * ns + kv)];
+            if (u >= 0 && kv < 3 * ns) pb[q] = phi[(unsigned)(j * 3 * ns + kv)];
           }
 #pragma unroll
           for (int i = 0; i < A * kPhiRegs; ++i) {
-            if (ub[i] < 0) break;
-            spread_atom(rec[b0 + ub[i]], pb[i / A], (i % A) * 32);
+            const int u = ub[i];
+            if (u < 0) break;
+            const int4 r{0, __builtin_amdgcn_readlane(mr.y, u), __builtin_amdgcn_readlane(mr.z, u),
+                         __builtin_amdgcn_readlane(mr.w, u)};
+            spread_atom(r, pb[i / A], (i % A) * 32);
           }
         }
       }
