@@ -776,6 +776,26 @@ __device__ __forceinline__ double row16_sum(double v) {
   return v;
 }
 
+// Lane K of each row of 16 lanes, broadcast to the row (DPP row_newbcast).
+template <int K> __device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + K, 0xF, 0xF, false));
+}
+template <int K> __device__ __forceinline__ double row_bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x150 + K, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + K, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// f(integral_constant<int, K>) for K = B .. E - 1, in order.
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>());
+    static_for<B + 1, E>(f);
+  }
+}
+
 // Lane shuffle through ds_bpermute with a byte address (4 * source lane)
 // computed once by the caller.
 __device__ __forceinline__ float bperm(int addr4, float v) {
@@ -1779,16 +1799,97 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
   const int S2 = g.sx * g.sy;
   // Gather: the tile is only read, so one wave serves several atoms at once:
   // lpa lanes per atom (one per xy column of its stencil in the tile, at
-  // least 8), ga = 64 / lpa atoms per round.  The kernel values of a round's
-  // atoms go to a per-wave scratch; each lane sums its column over z in the
-  // path precision, and the three derivative sums are reduced in FP64 over
-  // the atom's lanes.
+  // least 8), ga = 64 / lpa atoms per round.  Each lane sums its column over
+  // z in the path precision, and the three derivative sums are reduced in
+  // FP64 over the atom's lanes.
   const int lpa = max(8, txr * tty), ga = 64 / lpa;
   const int grp = lane / lpa, l = lane - grp * lpa;
+  // the (atom, tile) pair's slot: tiles counted from the one that holds the
+  // first stencil point, which the record places at tile column xll - kxl
+  // (row yll - kyl), modulo the grid
+  auto put_slot = [&](const int4 r, double gx, double gy, double gz) {
+    const int kxl = r.y & 255, xll = (r.y >> 8) & 255;
+    const int kyl = r.z & 255, yll = (r.z >> 8) & 255;
+    const int xs = wrapi(x0 + xll - kxl, n1), ys = wrapi(y0 + yll - kyl, n2);
+    const int dx = wrapi(tx - xs / txr, g.ntx);
+    const int dy = wrapi(ty - ys / tty, g.nty);
+    double *o = slots + ((size_t)r.x * S2 + dx * g.sy + dy) * 3;
+    o[0] = gx;
+    o[1] = gy;
+    o[2] = gz;
+  };
+  // The compiled 8 x 2 tile: 16 lanes per atom, one DPP row.  A round's
+  // kernel values stay in registers: lane i of an atom's row holds value i
+  // of each of the six tables (phi, then dphi, along x, y, z), and the next
+  // round's are read while this one computes.  A lane takes its column's x
+  // and y values by a shuffle within the row and the z values by row
+  // broadcasts, so a round touches the LDS only for the record and the
+  // tile column.
+  // (the record of the lane's atom comes with them)
+  auto fetch_rows = [&](T (&dst)[6], int4 &r, int s0, int na) {
+    r = rec[s0 + min(grp, na - 1)];
+    const bool ok = grp < na && l < ns;
+    const size_t b = (size_t)(ok ? r.x : 0) * 3 * ns + l;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      dst[q] = T(0);
+      if (ok) dst[q] = q < 3 ? phi[b + q * ns] : dphi[b + (q - 3) * ns];
+    }
+  };
+  auto gather_rows = [&](int nst) {
+    T cur[6], nxt[6];
+    int4 r{0, 0, 0, 0}, rn{0, 0, 0, 0};
+    if (wave * ga < nst) fetch_rows(cur, r, wave * ga, min(ga, nst - wave * ga));
+    for (int s0 = wave * ga; s0 < nst; s0 += NW * ga) {
+      const int na = min(ga, nst - s0);
+      const int s1 = s0 + NW * ga;
+      if (s1 < nst) fetch_rows(nxt, rn, s1, min(ga, nst - s1));
+      const bool act = grp < na;
+      const int kxl = r.y & 255, xll = (r.y >> 8) & 255, ox = r.y >> 16;
+      const int kyl = r.z & 255, yll = (r.z >> 8) & 255, oy = r.z >> 16;
+      const int a = r.w & 0xFFFF;
+      const bool cell = act && l < ox * oy;
+      const int lc = cell ? l : 0;
+      const int iy = small_div(lc, __builtin_amdgcn_rcpf((float)ox));
+      const int ix = lc - iy * ox;
+      // every lane takes part in the shuffles
+      const int ax = 4 * (16 * grp + kxl + ix), ay = 4 * (16 * grp + kyl + iy);
+      const T fx = bperm(ax, cur[0]), fy = bperm(ay, cur[1]);
+      const T dfx = bperm(ax, cur[3]), dfy = bperm(ay, cur[4]);
+      const int xl = xll + ix, yl = yll + iy;
+      // idle lanes read column 0 (in the tile, unused)
+      const T *colv = tflat + (cell ? 2 * (yl * ccx + (xl >> 1)) * cs + (xl & 1) : 0);
+      T sv = T(0), sd = T(0);
+      static_for<0, 16>([&](auto kc) {
+        constexpr int kz = decltype(kc)::value;
+        if (kz < ns) {
+          const T pz = row_bcast<kz>(cur[2]), dz = row_bcast<kz>(cur[5]);
+          const int z = a + kz < n3 ? a + kz : a + kz - n3;
+          const T v = colv[2 * z];
+          sv += v * pz;
+          sd += v * dz;
+        }
+      });
+      double gx = 0.0, gy = 0.0, gz = 0.0;
+      if (cell) {
+        gx = (double)(dfx * fy) * (double)sv;
+        gy = (double)(fx * dfy) * (double)sv;
+        gz = (double)(fx * fy) * (double)sd;
+      }
+      gx = row16_sum(gx);
+      gy = row16_sum(gy);
+      gz = row16_sum(gz);
+      if (act && l == 0) put_slot(r, gx, gy, gz);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) cur[q] = nxt[q];
+      r = rn;
+    }
+  };
+  // Other tiles: the kernel values of a round's atoms go to a per-wave
+  // scratch, [atom][phi 3 ns, dphi 3 ns]: value k of the round is read by
+  // lane k % 64 into register k / 64.  When they fit kGV registers, the next
+  // round's are read while this one computes.
   T *scr = reinterpret_cast<T *>(rec + kStageCap) + (size_t)wave * ga * 6 * ns;
-  // The kernel values of a round's atoms, [atom][phi 3 ns, dphi 3 ns]: value
-  // k of the round is read by lane k % 64 into register k / 64.  When they
-  // fit kGV registers, the next round's are read while this one computes.
   constexpr int kGV = 6;
   const int nval = 6 * ns;
   const float inv_nval = __builtin_amdgcn_rcpf((float)nval);
@@ -1806,10 +1907,7 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
       }
     }
   };
-  for (int base = 0; base < st.total; base += NT) {
-    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0s);
-    const bool last = base + NT >= st.total;
-    if (!last && nst <= kStageCap - NT) continue;
+  auto gather_scratch = [&](int nst) {
     if (ahead && wave * ga < nst) fetch(wave * ga, min(ga, nst - wave * ga));
     for (int s0 = wave * ga; s0 < nst; s0 += NW * ga) {
       const int na = min(ga, nst - s0);
@@ -1873,21 +1971,18 @@ __global__ void __launch_bounds__(NT) k_adj_z_gather(Geo g, AxisPlan az,
           gz += __shfl_xor(gz, off);
         }
       }
-      if (act && l == 0) {
-        // slot of this (atom, tile) pair: tiles counted from the one that
-        // holds the first stencil point, which the record places at tile
-        // column xll - kxl (row yll - kyl), modulo the grid
-        const int j = r.x;
-        const int xs = wrapi(x0 + xll - kxl, n1), ys = wrapi(y0 + yll - kyl, n2);
-        const int dx = wrapi(tx - xs / txr, g.ntx);
-        const int dy = wrapi(ty - ys / tty, g.nty);
-        double *o = slots + ((size_t)j * S2 + dx * g.sy + dy) * 3;
-        o[0] = gx;
-        o[1] = gy;
-        o[2] = gz;
-      }
+      if (act && l == 0) put_slot(r, gx, gy, gz);
       wave_sync();  // the next round overwrites the scratch
     }
+  };
+  for (int base = 0; base < st.total; base += NT) {
+    const int nst = stage_chunk(st, rec, base, g, x0, txw, y0, tyw, sidx, i0s);
+    const bool last = base + NT >= st.total;
+    if (!last && nst <= kStageCap - NT) continue;
+    if constexpr (kt)
+      gather_rows(nst);
+    else
+      gather_scratch(nst);
     __syncthreads();
     if (tid == 0) st.nstaged = 0;
     __syncthreads();
