@@ -125,6 +125,17 @@ int sqc_hip_plan_create(sqc_hip_plan **plan, const sqc_hip_config *cfg);
 
 int sqc_hip_plan_info(const sqc_hip_plan *plan, sqc_hip_info *info);
 
+/* Pin the launch shapes the autotuner chooses (sqc_hip_info.tile and
+ * x_lines): spread tile spread_x x spread_y real columns (16x2, 8x2, 16x1,
+ * 4x2, 8x1, 4x1 or 2x1, as the plan's LDS and bin window allow), y-pass
+ * column chunk 2, 4, 8 or 16, x-pass lines per group 1..8.  The plan then
+ * skips tuning.  Shapes the plan cannot run return SQC_HIP_ERR_BAD_ARG and
+ * leave it unchanged.  Waits for the plan's queued work.  Counterpart of
+ * the cuFINUFFT spreader options gpu_binsizex/y/z and gpu_maxsubprobsize
+ * the reference passes with its plan (src/sqc_cufinufft.f90:45-53), which
+ * it leaves at their defaults (:281-289). */
+int sqc_hip_set_tiles(sqc_hip_plan *plan, int spread_x, int spread_y, int y_chunk, int x_lines);
+
 /* Shell bookkeeping of the plan (den and shell_count of
  * src/sqc_structure_factor.f90:408-416). */
 int sqc_hip_shells(const sqc_hip_plan *plan, double *den, int64_t *shell_count);

@@ -105,6 +105,7 @@ SIGNATURES = {
     "sqc_hip_device_count": (_c.c_int, [_c.POINTER(_c.c_int)]),
     "sqc_hip_plan_create": (_c.c_int, [_c.POINTER(_vp), _c.POINTER(Config)]),
     "sqc_hip_plan_info": (_c.c_int, [_vp, _c.POINTER(Info)]),
+    "sqc_hip_set_tiles": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     "sqc_hip_shells": (_c.c_int, [_vp, _vp, _vp]),
     "sqc_hip_modes": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "sqc_hip_accumulate": (_c.c_int, [_vp, _vp, _c.c_int64]),

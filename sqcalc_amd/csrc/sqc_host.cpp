@@ -1227,6 +1227,31 @@ int sqc_hip_plan_info(const sqc_hip_plan *plan, sqc_hip_info *info) {
   return SQC_HIP_OK;
 }
 
+int sqc_hip_set_tiles(sqc_hip_plan *plan, int spread_x, int spread_y, int y_chunk,
+                      int x_lines) {
+  if (int rc = check_plan(plan)) return rc;
+  sqc::Plan &p = plan->p;
+  bool listed = false;
+  for (int c = 0; c < kNumTiles; ++c) listed |= kTiles[c][0] == spread_x && kTiles[c][1] == spread_y;
+  if (!listed || !tile_fits(p, spread_x, spread_y, kLdsCap))
+    return fail(SQC_HIP_ERR_BAD_ARG, "spread tile " + std::to_string(spread_x) + " x " +
+                                         std::to_string(spread_y) + " is not usable for this plan");
+  if ((y_chunk != 2 && y_chunk != 4 && y_chunk != 8 && y_chunk != 16) ||
+      ypass_lds(p, y_chunk) > kLdsCap)
+    return fail(SQC_HIP_ERR_BAD_ARG, "y-pass column chunk " + std::to_string(y_chunk) +
+                                         " is not usable for this plan");
+  if (x_lines < 1 || x_lines > sqc::kMaxLb || xpass_lds(p, x_lines) > xpass_cap())
+    return fail(SQC_HIP_ERR_BAD_ARG, "x-pass line group " + std::to_string(x_lines) +
+                                         " is not usable for this plan");
+  HIP_TRY(hipSetDevice(p.device), "hipSetDevice");
+  // the buffers the tiles size are reallocated: nothing queued may use them
+  HIP_TRY(hipStreamSynchronize(p.stream), "hipStreamSynchronize");
+  if (p.stream2) HIP_TRY(hipStreamSynchronize(p.stream2), "hipStreamSynchronize");
+  if (int rc = apply_tiles(p, spread_x, spread_y, y_chunk, x_lines)) return rc;
+  p.tuned = 1;  // pinned: the tuner does not run on the next frame
+  return SQC_HIP_OK;
+}
+
 int sqc_hip_shells(const sqc_hip_plan *plan, double *den, int64_t *shell_count) {
   if (int rc = check_plan(plan)) return rc;
   const sqc::Plan &p = plan->p;

@@ -192,6 +192,14 @@ class HipStructureFactor:
     def reset(self):
         _lib.check(self.lib.sqc_hip_reset(self.plan), "reset")
 
+    def set_tiles(self, spread_x, spread_y, y_chunk, x_lines):
+        """Pin the launch shapes the autotuner would choose (see ``info.tile``
+        and ``info.x_lines``); the plan then skips tuning.  Raises for shapes
+        the plan cannot run (the cuFINUFFT gpu_binsize options of
+        src/sqc_cufinufft.f90:45-53)."""
+        _lib.check(self.lib.sqc_hip_set_tiles(self.plan, int(spread_x), int(spread_y),
+                                              int(y_chunk), int(x_lines)), "set_tiles")
+
     def profile(self, enable=True):
         """Record HIP events around every stage launch (see stage_times)."""
         _lib.check(self.lib.sqc_hip_profile(self.plan, 1 if enable else 0), "profile")
