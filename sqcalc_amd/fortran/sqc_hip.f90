@@ -74,8 +74,8 @@ module sqc_hip
       integer(c_int32_t) :: x_lines
    end type sqc_hip_info_t
 
-   public :: sqc_hip_device_count, sqc_hip_plan_create, sqc_hip_plan_info, sqc_hip_shells, &
-             sqc_hip_modes, sqc_hip_accumulate, sqc_hip_fetch, sqc_hip_fetch_mode_partials, sqc_hip_reset, &
+   public :: sqc_hip_device_count, sqc_hip_plan_create, sqc_hip_plan_info, sqc_hip_set_tiles, &
+             sqc_hip_shells, sqc_hip_modes, sqc_hip_accumulate, sqc_hip_fetch, sqc_hip_fetch_mode_partials, sqc_hip_reset, &
              sqc_hip_gradient, sqc_hip_destroy, sqc_hip_config_size
    public :: sqc_hip_message, sqc_hip_detail, sqc_hip_layout_ok
 
@@ -96,6 +96,15 @@ module sqc_hip
          type(c_ptr), value :: plan
          type(sqc_hip_info_t), intent(out) :: info
       end function sqc_hip_plan_info
+
+      ! pin the spread tile, y-pass column chunk and x-pass line group (the
+      ! cuFINUFFT gpu_binsize options of src/sqc_cufinufft.f90:45-53)
+      integer(c_int) function sqc_hip_set_tiles(plan, spread_x, spread_y, y_chunk, x_lines) &
+         bind(C, name='sqc_hip_set_tiles')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: plan
+         integer(c_int), value :: spread_x, spread_y, y_chunk, x_lines
+      end function sqc_hip_set_tiles
 
       integer(c_int) function sqc_hip_shells(plan, den, shell_count) &
          bind(C, name='sqc_hip_shells')

@@ -9,7 +9,8 @@
 !!                     [--precision double|single] [--norm mean|self|n]
 !!                     [--partials|--no-partials] [-fz] [--gpu-id K]
 !!                     [--grid FILE] [--xrd FILE --xrd-lambda L [--xrd-range MIN MAX]
-!!                     [--xrd-step S] [--lp|--no-lp]] [--tune] [--quiet] [OUT|-]
+!!                     [--xrd-step S] [--lp|--no-lp]] [--tune | --tiles X,Y,C,L] [--quiet]
+!!                     [OUT|-]
 program sqcalc_hip
    use, intrinsic :: iso_c_binding
    use, intrinsic :: iso_fortran_env, only: error_unit, output_unit, real64, int64
@@ -29,6 +30,8 @@ program sqcalc_hip
    real(rk) :: qmin = 0.0_rk, qmax = 20.0_rk, eps = 1.0e-9_rk
    logical :: eps_given = .false., partials = .true., faber_ziman = .false., quiet = .false.
    logical :: tune = .false., want_grid = .false.
+   ! --tiles X,Y,C,L: pinned spread tile, y-pass chunk and x-pass line group
+   integer(c_int) :: tiles(4) = 0
    logical :: qmin_given = .false., qmax_given = .false., weight_given = .false.
    ! powder XRD (src/sqc_options.f90:90-104, :620-660)
    character(len=:), allocatable :: xrd_output
@@ -115,6 +118,13 @@ program sqcalc_hip
    if (rc /= sqc_hip_ok .and. rc /= sqc_hip_warn_eps_too_small) then
       write (error_unit, '(a)') 'sqcalc_hip: '//sqc_hip_message(rc)//': '//sqc_hip_detail()
       stop 6
+   end if
+   if (tiles(1) > 0) then
+      rc = sqc_hip_set_tiles(plan, tiles(1), tiles(2), tiles(3), tiles(4))
+      if (rc /= sqc_hip_ok) then
+         write (error_unit, '(a)') 'sqcalc_hip: '//sqc_hip_message(rc)//': '//sqc_hip_detail()
+         stop 6
+      end if
    end if
    rc = sqc_hip_plan_info(plan, info)
    if (.not. quiet) call report()
@@ -582,7 +592,7 @@ contains
       write (u, '(a)') 'usage: sqcalc_hip static -i DUMP [-m 1:Si,2:O] [-w unit|neutron|xray]'
       write (u, '(a)') '         [--qmin Q] [--qmax Q] [--nq N] [--eps E] [--precision double|single]'
       write (u, '(a)') '         [--norm mean|self|n] [--partials|--no-partials] [-fz] [--gpu-id K]'
-      write (u, '(a)') '         [--device gpu] [--grid FILE] [--tune] [--quiet] [OUT|-]'
+      write (u, '(a)') '         [--device gpu] [--grid FILE] [--tune | --tiles X,Y,C,L] [--quiet] [OUT|-]'
       write (u, '(a)') '         [--xrd FILE --xrd-lambda L [--xrd-range MIN MAX] [--xrd-step S]'
       write (u, '(a)') '          [--lp|--no-lp]]'
    end subroutine usage
@@ -720,7 +730,7 @@ contains
             xrd_opts = .true.
          case ('-i', '--input', '-m', '--mapping', '-w', '--weight', '--qmin', '--qmax', '--nq', &
                '--eps', '--precision', '--device', '--gpu-id', '--norm', '--grid', '--xrd', &
-               '--xrd-lambda', '--xrd-step')
+               '--xrd-lambda', '--xrd-step', '--tiles')
             if (.not. inline) then
                i = i + 1
                if (i > n) call fail_usage(name//' needs a value')
@@ -777,6 +787,11 @@ contains
                case default
                   call fail_usage('unknown device "'//value//'" (use gpu)')
                end select
+            case ('--tiles')
+               ! spread tile x, y, y-pass column chunk, x-pass lines per group
+               read (value, *, iostat=ios) tiles
+               if (ios /= 0 .or. any(tiles < 1)) &
+                  call fail_usage('--tiles takes four positive integers: X,Y,CHUNK,LINES')
             case ('--gpu-id')
                read (value, *, iostat=ios) gpu_id
                if (ios /= 0 .or. gpu_id < 0) call fail_usage('GPU id must be a non-negative integer')

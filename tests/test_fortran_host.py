@@ -90,6 +90,8 @@ def test_species_without_data(exe, tmp_path, weight, label, text):
      'element "Fe" has no charge state "Fe7+"; available: Fe, Fe2+, Fe3+'),
     (["static", "-i", "x.dump", "--qmin", "5", "--qmax", "4"], 2, "larger than qmin"),
     (["static", "-i", "x.dump", "--precision", "half"], 2, "unknown precision"),
+    (["static", "-i", "x.dump", "--tiles", "8,2"], 2, "--tiles takes four positive integers"),
+    (["static", "-i", "x.dump", "--tiles", "8,2,0,1"], 2, "--tiles takes four positive integers"),
     (["static", "-i", "/nonexistent/x.dump"], 4, "cannot open"),
     # --xrd rules of src/sqc_options.f90:620-660
     (["static", "-i", "x.dump", "--xrd", "p.txt"], 2, "--xrd-lambda"),
@@ -129,16 +131,20 @@ def test_setup_without_device(exe, tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", golden_cases())
-def test_golden_tables_through_fortran(exe, hip_device, tmp_path, name):
+@pytest.mark.parametrize("tiles", [None, "4,2,4,1"])
+def test_golden_tables_through_fortran(exe, hip_device, tmp_path, name, tiles):
+    """Default launch shapes, and pinned ones (--tiles: spread tile 4 x 2,
+    y-pass chunk 4, one x line per group)."""
     d = load_golden(name)
     scaled = name.startswith("tri")
     p = tmp_path / (name + ".dump")
     write_dump(p, d["frames"], d["types"], d["lo"], d["hi"], d["tilt"], scaled=scaled)
     out = tmp_path / "sq.txt"
-    print("case fortran golden", name, flush=True)
+    print("case fortran golden", name, tiles, flush=True)
     r = run(["static", "-i", str(p), "-m", d["mapping"], "-w", d["weight"], "--norm", d["norm"],
              "--qmin", repr(d["qmin"]), "--qmax", repr(d["qmax"]), "--nq", str(d["nq"]),
-             "--eps", "1e-10", "--no-partials", "--quiet", str(out)])
+             "--eps", "1e-10", "--no-partials", "--quiet", str(out)]
+            + (["--tiles", tiles] if tiles else []))
     assert r.returncode == 0, r.stderr
     header, rows = read_table(out.read_text())
     assert header.strip() == "# q S(q)"
