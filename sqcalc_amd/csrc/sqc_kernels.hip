@@ -973,14 +973,14 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
         const int idx = b0 + lane;
         const int4 mr = idx < nst ? rec[idx] : int4{0, 0, 0, 0};
         unsigned long long m = __ballot((mr.w >> (24 + wave)) & 1);
-        while (m) {
-          int ub[A * kPhiRegs];
+        // the next group's atoms (lanes of mr) and their kernel values: a
+        // group's loads are issued before the previous group is spread
+        auto take = [&](int (&ub)[A * kPhiRegs], T (&pb)[kPhiRegs]) {
 #pragma unroll
           for (int i = 0; i < A * kPhiRegs; ++i) {
             ub[i] = m ? __ffsll((long long)m) - 1 : -1;
             m &= m - 1;
           }
-          T pb[kPhiRegs];
 #pragma unroll
           for (int q = 0; q < kPhiRegs; ++q) {
             const int u0 = ub[A * q], u1 = ub[A * q + A - 1];
@@ -993,14 +993,27 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
             // 32-bit offsets: natoms * 3 ns < 2^31 (checked by the plan)
             if (u >= 0 && kv < 3 * ns) pb[q] = phi[(unsigned)(j * 3 * ns + kv)];
           }
+        };
+        if (!m) continue;
+        int ub[A * kPhiRegs], un[A * kPhiRegs];
+        T pb[kPhiRegs], pn[kPhiRegs];
+        take(ub, pb);
+        while (true) {
+          const bool more = m != 0;
+          if (more) take(un, pn);
+          // the taken atoms come first in ub, the rest are -1
 #pragma unroll
           for (int i = 0; i < A * kPhiRegs; ++i) {
             const int u = ub[i];
-            if (u < 0) break;
-            const int4 r{0, __builtin_amdgcn_readlane(mr.y, u), __builtin_amdgcn_readlane(mr.z, u),
-                         __builtin_amdgcn_readlane(mr.w, u)};
-            spread_atom(r, pb[i / A], (i % A) * 32);
+            if (u >= 0) {
+              const int4 r{0, __builtin_amdgcn_readlane(mr.y, u),
+                           __builtin_amdgcn_readlane(mr.z, u), __builtin_amdgcn_readlane(mr.w, u)};
+              spread_atom(r, pb[i / A], (i % A) * 32);
+            }
           }
+          if (!more) break;
+          for (int i = 0; i < A * kPhiRegs; ++i) ub[i] = un[i];
+          for (int q = 0; q < kPhiRegs; ++q) pb[q] = pn[q];
         }
       }
     };
