@@ -566,24 +566,33 @@ __global__ void k_binsort(int nb, const int32_t *__restrict__ start, int32_t *__
     }
     if (lane < n) {
       sidx[k0 + rank] = v;
-      i0s[k0 + rank] = i0[v];  // the stencil corners in sorted order
+      int4 c = i0[v];  // the stencil corners in sorted order, .w the atom
+      c.w = v;
+      i0s[k0 + rank] = c;
     }
     return;
   }
-  if (lane == 0) {
-    for (int k = k0 + 1; k < k1; ++k) {
-      const int v = sidx[k];
-      int i = k - 1;
-      while (i >= k0 && sidx[i] > v) {
-        sidx[i + 1] = sidx[i];
-        --i;
-      }
-      sidx[i + 1] = v;
+  // larger bins: the same ranks, counted 64 candidates at a time, with the
+  // corners written to i0s in sorted order (.w holds the atom index, which
+  // then goes back to sidx), so no element is overwritten before it is read
+  for (int e0 = 0; e0 < n; e0 += 64) {
+    const int e = e0 + lane;
+    const int v = e < n ? sidx[k0 + e] : 0x7fffffff;
+    int rank = 0;
+    for (int s0 = 0; s0 < n; s0 += 64) {
+      const int w = s0 + lane < n ? sidx[k0 + s0 + lane] : 0x7fffffff;
+      const int m = min(64, n - s0);
+      for (int i = 0; i < m; ++i) rank += (__shfl(w, i) < v) ? 1 : 0;
+    }
+    if (e < n) {
+      int4 c = i0[v];
+      c.w = v;
+      i0s[k0 + rank] = c;
     }
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  for (int k = k0 + lane; k < k1; k += 64) i0s[k] = i0[sidx[k]];
+  for (int k = k0 + lane; k < k1; k += 64) sidx[k] = i0s[k].w;
 }
 
 // Bins to visit around a tile, without visiting a bin twice on small grids.
