@@ -730,6 +730,8 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   g.ns = ns;
   g.beta = bon * ns;
   g.half_ns = 0.5 * ns;
+  g.inv_half_ns = 2.0 / ns;
+  g.inv_per = 1.0 / (3.0 * ns);
   for (int i = 0; i < 3; ++i) {
     const int64_t nf = next235even(std::max<int64_t>((int64_t)std::ceil(sigma * p.m[i]), 2 * ns));
     if (nf > 16384) return fail(SQC_HIP_ERR_MAXNALLOC, "fine grid exceeds 16384 points per axis");

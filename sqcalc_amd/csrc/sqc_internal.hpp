@@ -73,6 +73,8 @@ struct Geo {
   int ns;
   double beta;
   double half_ns;
+  double inv_half_ns;  // 2 / ns
+  double inv_per;      // 1 / (3 ns): atom of a kernel-value thread
   // spread tiles (real x-columns x rows) and the bins that sort atoms by tile
   int txr, ty, ntx, nty, nbins;
   int ltxr, lty;     // log2 of txr and ty

@@ -345,8 +345,8 @@ template <> __device__ __forceinline__ float es_exp<float>(double a) { return __
 template <> __device__ __forceinline__ double es_exp<double>(double a) { return exp(a); }
 
 template <typename T>
-__device__ __forceinline__ T es_phi(double u, double ns, double beta, T *dphi) {
-  const double z = 2.0 * u / ns;
+__device__ __forceinline__ T es_phi(double u, double inv_half_ns, double beta, T *dphi) {
+  const double z = u * inv_half_ns;  // 2 u / ns
   const double q = 1.0 - z * z;
   if (q <= 0.0) {
     if (dphi) *dphi = T(0);
@@ -354,7 +354,7 @@ __device__ __forceinline__ T es_phi(double u, double ns, double beta, T *dphi) {
   }
   const double sq = sqrt(q);
   const T ph = es_exp<T>(beta * (sq - 1.0));
-  if (dphi) *dphi = (T)((double)ph * (-beta * z / sq) * (2.0 / ns));
+  if (dphi) *dphi = (T)((double)ph * (-beta * z / sq) * inv_half_ns);
   return ph;
 }
 
@@ -435,9 +435,15 @@ __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__r
                        int32_t *__restrict__ key, int32_t *__restrict__ count) {
   const int ns = g.ns, per = 3 * ns;
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;  // < natoms * 3 ns < 2^31
-  const unsigned j = t / (unsigned)per;
+  // j = t / per and ia = v / ns by reciprocals: the FP64 quotient of
+  // t < 2^31 is off by at most one and corrected; v < 48 divides exactly
+  // in FP32
+  unsigned j = (unsigned)((double)t * g.inv_per);
+  if (j * (unsigned)per > t) --j;
+  else if ((j + 1) * (unsigned)per <= t) ++j;
   if (j >= (unsigned)g.natoms) return;
-  const int v = (int)(t - j * per), ia = v / ns, k = v - ia * ns;
+  const int v = (int)(t - j * per);
+  const int ia = small_div(v, (float)g.inv_half_ns * 0.5f), k = v - ia * ns;
   const double d0 = pos[3 * (size_t)j] - g.origin[0];
   const double d1 = pos[3 * (size_t)j + 1] - g.origin[1];
   const double d2 = pos[3 * (size_t)j + 2] - g.origin[2];
@@ -452,7 +458,7 @@ __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__r
   const double u = grid_coord(ia);
   const int i0 = (int)ceil(u - g.half_ns);
   T d;
-  phi[t] = es_phi<T>((double)(i0 + k) - u, (double)ns, g.beta, dphi ? &d : (T *)nullptr);
+  phi[t] = es_phi<T>((double)(i0 + k) - u, g.inv_half_ns, g.beta, dphi ? &d : (T *)nullptr);
   if (dphi) dphi[t] = d;
   if (v == 0) {
     double x[3];
