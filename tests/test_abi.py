@@ -49,6 +49,9 @@ def test_error_strings_and_bad_arguments():
     cfg.precision = 7
     assert lib.sqc_hip_plan_create(ctypes.byref(plan), ctypes.byref(cfg)) == 20
     assert not plan.value
+    # calls on no plan are refused before anything touches a device
+    assert lib.sqc_hip_set_tiles(None, 8, 2, 8, 2) == 20
+    assert b"plan" in lib.sqc_hip_last_error()
 
 
 def test_product_has_no_cpu_fallback(monkeypatch):
