@@ -982,6 +982,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
     // together (A atoms per register, at lanes 0 and 32 when A = 2)
     auto walk = [&](auto per_reg) {
       constexpr int A = decltype(per_reg)::value;
+      // registers of kernel values per group: eight atoms a group either way
+      constexpr int PR = A == 2 ? kPhiRegs : 2 * kPhiRegs;
       for (int b0 = 0; b0 < nst; b0 += 64) {
         // every lane holds one record of the block; an atom's fields are
         // read from its lane (no LDS round trip per atom)
@@ -990,14 +992,14 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
         unsigned long long m = __ballot((mr.w >> (24 + wave)) & 1);
         // the next group's atoms (lanes of mr) and their kernel values: a
         // group's loads are issued before the previous group is spread
-        auto take = [&](int (&ub)[A * kPhiRegs], T (&pb)[kPhiRegs]) {
+        auto take = [&](int (&ub)[A * PR], T (&pb)[PR]) {
 #pragma unroll
-          for (int i = 0; i < A * kPhiRegs; ++i) {
+          for (int i = 0; i < A * PR; ++i) {
             ub[i] = m ? __ffsll((long long)m) - 1 : -1;
             m &= m - 1;
           }
 #pragma unroll
-          for (int q = 0; q < kPhiRegs; ++q) {
+          for (int q = 0; q < PR; ++q) {
             const int u0 = ub[A * q], u1 = ub[A * q + A - 1];
             const int j0 = u0 >= 0 ? __builtin_amdgcn_readlane(mr.x, u0) : 0;
             const int j1 = u1 >= 0 ? __builtin_amdgcn_readlane(mr.x, u1) : 0;
@@ -1010,15 +1012,15 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
           }
         };
         if (!m) continue;
-        int ub[A * kPhiRegs], un[A * kPhiRegs];
-        T pb[kPhiRegs], pn[kPhiRegs];
+        int ub[A * PR], un[A * PR];
+        T pb[PR], pn[PR];
         take(ub, pb);
         while (true) {
           const bool more = m != 0;
           if (more) take(un, pn);
           // the taken atoms come first in ub, the rest are -1
 #pragma unroll
-          for (int i = 0; i < A * kPhiRegs; ++i) {
+          for (int i = 0; i < A * PR; ++i) {
             const int u = ub[i];
             if (u >= 0) {
               const int4 r{0, __builtin_amdgcn_readlane(mr.y, u),
@@ -1027,8 +1029,8 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
             }
           }
           if (!more) break;
-          for (int i = 0; i < A * kPhiRegs; ++i) ub[i] = un[i];
-          for (int q = 0; q < kPhiRegs; ++q) pb[q] = pn[q];
+          for (int i = 0; i < A * PR; ++i) ub[i] = un[i];
+          for (int q = 0; q < PR; ++q) pb[q] = pn[q];
         }
       }
     };
