@@ -7,7 +7,10 @@ with N(0, 0.3^2) jitter per frame, neutron weights, partials on, q_max 10,
 500 shells.  Positions are generated from a seed and resident in HBM before
 the timed region.
 
-A step is one pass of the path over a batch of --frames-per-step frames.
+A step is one pass of the path over a batch of --frames-per-step frames
+(one captured graph replay; 25 frames by default, a quarter of the 100-frame
+trajectory, so the pipeline of each replay fills and drains once per 25
+frames).
 The headline value is the FP32-spread forward (S(q) + partials, FP64 shell
 sums) in frames/s.  Next to it the line reports the plain FP64 (sigma = 2)
 forward, the forward + adjoint of the S(q)-matching loss on the same
@@ -17,7 +20,7 @@ host cores.  An "ingest" object times frames read from a LAMMPS dump of the
 same workload: the native reader alone, dump -> S(q) end to end, and the
 reference's own reader on the same file.
 
-  python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--frames-per-step 10]
+  python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--frames-per-step 25]
 """
 
 import argparse
@@ -45,7 +48,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--frames-per-step", type=int, default=10)
+    ap.add_argument("--frames-per-step", type=int, default=25)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip forward + adjoint")
