@@ -732,6 +732,14 @@ int build_plan(sqc_hip_plan *h, const sqc_hip_config *cfg) {
   g.half_ns = 0.5 * ns;
   g.inv_half_ns = 2.0 / ns;
   g.inv_per = 1.0 / (3.0 * ns);
+  // dphi/du = -phi beta z / sqrt(1 - z^2) (2 / ns) has an integrable
+  // singularity at the support edge |z| = 1, where phi = e^-beta.  An atom a
+  // few ulps inside the edge (atoms on grid points or half-points of a
+  // lattice-aligned cell) would get a spike of e^-beta beta / sqrt(1 - z^2)
+  // there.  Bounding the square root below by 2 e^-beta / eps keeps that
+  // term under eps beta / ns, the transform's own accuracy relative to the
+  // derivative's scale; elsewhere dphi is unchanged.
+  g.dphi_smin = std::min(0.1, std::max(1e-6, 2.0 * std::exp(-g.beta) / eps));
   for (int i = 0; i < 3; ++i) {
     const int64_t nf = next235even(std::max<int64_t>((int64_t)std::ceil(sigma * p.m[i]), 2 * ns));
     if (nf > 16384) return fail(SQC_HIP_ERR_MAXNALLOC, "fine grid exceeds 16384 points per axis");

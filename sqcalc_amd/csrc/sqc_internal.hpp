@@ -75,6 +75,7 @@ struct Geo {
   double half_ns;
   double inv_half_ns;  // 2 / ns
   double inv_per;      // 1 / (3 ns): atom of a kernel-value thread
+  double dphi_smin;    // lower bound on sqrt(1 - z^2) in the kernel derivative
   // spread tiles (real x-columns x rows) and the bins that sort atoms by tile
   int txr, ty, ntx, nty, nbins;
   int ltxr, lty;     // log2 of txr and ty

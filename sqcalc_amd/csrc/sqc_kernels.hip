@@ -344,8 +344,11 @@ template <typename T> __device__ __forceinline__ T es_exp(double a);
 template <> __device__ __forceinline__ float es_exp<float>(double a) { return __expf((float)a); }
 template <> __device__ __forceinline__ double es_exp<double>(double a) { return exp(a); }
 
+// dphi bounds sqrt(1 - z^2) below by smin, which caps the derivative's
+// singular term at the support edge.
 template <typename T>
-__device__ __forceinline__ T es_phi(double u, double inv_half_ns, double beta, T *dphi) {
+__device__ __forceinline__ T es_phi(double u, double inv_half_ns, double beta, double smin,
+                                    T *dphi) {
   const double z = u * inv_half_ns;  // 2 u / ns
   const double q = 1.0 - z * z;
   if (q <= 0.0) {
@@ -354,7 +357,7 @@ __device__ __forceinline__ T es_phi(double u, double inv_half_ns, double beta, T
   }
   const double sq = sqrt(q);
   const T ph = es_exp<T>(beta * (sq - 1.0));
-  if (dphi) *dphi = (T)((double)ph * (-beta * z / sq) * inv_half_ns);
+  if (dphi) *dphi = (T)((double)ph * (-beta * z / fmax(sq, smin)) * inv_half_ns);
   return ph;
 }
 
@@ -458,7 +461,8 @@ __global__ void k_prep(Geo g, const double *__restrict__ pos, const int32_t *__r
   const double u = grid_coord(ia);
   const int i0 = (int)ceil(u - g.half_ns);
   T d;
-  phi[t] = es_phi<T>((double)(i0 + k) - u, g.inv_half_ns, g.beta, dphi ? &d : (T *)nullptr);
+  phi[t] = es_phi<T>((double)(i0 + k) - u, g.inv_half_ns, g.beta, g.dphi_smin,
+                     dphi ? &d : (T *)nullptr);
   if (dphi) dphi[t] = d;
   if (v == 0) {
     double x[3];
