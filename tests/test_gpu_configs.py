@@ -155,6 +155,26 @@ def test_c2_perfect_lattice_bragg(hip_device, precision, tol):
     dev = rel_dev(ref, got)
     print("  %d Bragg modes in %d shells, max deviation %.2e" % (nb.sum(), (nb > 0).sum(), dev))
     assert dev < tol
+
+    # dL/dr of L = sum_s (S_s - S_s / 2)^2 vanishes on a perfect Bravais
+    # lattice: e^{i q r_j} = 1 at every Bragg mode, so each mode's term
+    # 2 Re(conj(rho) i q e^{i q r_j}) / den = Re(2 i N q) / den is 0, and
+    # rho = 0 at every other mode.  The residue is held to the gradient
+    # tolerance (1e-8 / 1e-5) times the sum of the magnitudes that cancel,
+    # sum over Bragg modes of |dL/dS_s| 2 |q| / count_s (den = N count_s).
+    import torch
+
+    from sqcalc_amd.autograd import structure_factor
+    pt = torch.tensor(pos, device=hip_device, requires_grad=True)
+    S, _ = structure_factor(pt, sf)
+    loss = torch.sum((S - 0.5 * S.detach()) ** 2)
+    loss.backward()
+    g = np.abs(pt.grad.cpu().numpy()).max()
+    gS = np.abs(S.detach().cpu().numpy())
+    qb = np.linalg.norm(qvec[bragg], axis=1)
+    scale = float(np.sum(gS[shell] * 2 * qb / sf.shell_count[shell]))
+    print("  lattice gradient: max |dL/dr| = %.2e of %.2e" % (g, scale))
+    assert g <= (1e-8 if precision == "double" else 1e-5) * scale
     sf.close()
 
 
