@@ -150,3 +150,56 @@ def test_one_atom_closed_form(hip_device, precision):
         assert zero == 0.0
         assert g <= G_TOL[precision] * 2 * int(has.sum()) * 5.0
     sf.close()
+
+
+CELLS = {
+    # tilts at LAMMPS's limit of half a box length
+    "skewed": (np.array([[24.0, 12.0, -12.0], [0.0, 22.0, 11.0], [0.0, 0.0, 20.0]]), 1500, 5.0),
+    # a 2 ns fine grid: every stencil covers half an axis, spans several z
+    # slabs and wraps through 0
+    "tiny": (np.array([[4.1, 0.3, 0.0], [0.0, 3.9, -0.2], [0.0, 0.0, 4.3]]), 9, 9.0),
+}
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("name", list(CELLS))
+def test_cell_shapes_forward_and_adjoint(hip_device, precision, name):
+    """A cell skewed to LAMMPS's tilt limit, and a cell so small that the fine
+    grid sits at its 2 ns floor: S(q), the pair columns and dL/dr against the
+    oracle, at the tolerances above."""
+    import torch
+
+    from sqcalc_amd.autograd import structure_factor
+    cell, natoms, qmax = CELLS[name]
+    nq, nsp = 30, 2
+    rng = np.random.default_rng(21)
+    types = rng.integers(1, 3, natoms)
+    types[:2] = [1, 2]
+    pos0 = (rng.random((natoms, 3)) * 1.4 - 0.2) @ cell.T
+    from sqcalc_amd import HipStructureFactor
+    sf = HipStructureFactor(cell, np.zeros(3), types, MAPPING, weight="xray", norm="mean",
+                            qmax=qmax, nq=nq, precision=precision, partials=True,
+                            eps=EPS[precision])
+    print("case cell", name, precision, "fine", list(sf.info.fine), "modes", sf.info.nmodes,
+          "ns", sf.info.nspread, flush=True)
+    target = rng.random(nq)
+    pos = torch.tensor(pos0, device=hip_device, requires_grad=True)
+    S, P = structure_factor(pos, sf)
+    wP = rng.standard_normal(tuple(P.shape))
+    loss = (torch.sum((S - torch.tensor(target, device=hip_device)) ** 2)
+            + torch.sum(torch.tensor(wP, device=hip_device) * P))
+    loss.backward()
+    grad = pos.grad.cpu().numpy()
+
+    run = O.Run(cell, np.zeros(3), types, MAPPING, weight="xray", norm="mean", qmax=qmax, nq=nq,
+                partials=True)
+    assert len(run.md["qlen"]) == sf.info.nmodes
+    gP = np.zeros((nsp, nsp, nq))
+    gP[0, 0], gP[0, 1], gP[1, 1] = wP[0], wP[1], wP[2]
+    Sref, gref = _oracle(pos0, cell, types, run, nsp, gS=lambda s: 2 * (s - target), gP=gP)
+    dS = float(np.abs(S.detach().cpu().numpy() - Sref).max())
+    err = float(np.abs(grad - gref).max() / np.abs(gref).max())
+    print("  max |dS| = %.2e, max |dg| / max |g| = %.2e" % (dS, err), flush=True)
+    assert dS <= S_TOL[precision]
+    assert err <= G_TOL[precision]
+    sf.close()
