@@ -34,6 +34,19 @@ def test_adjoint_matches_oracle(hip_device, precision, weight, partials, nspecie
     any-field-count instances, with the weights read from the plan's table
     (and, for one weighted field, the spreader's and the slot sum's
     per-species strengths as well)."""
+    _adjoint_case(hip_device, precision, weight, partials, nspecies, "mean")
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("norm", ["self", "n"])
+def test_adjoint_other_norms(hip_device, precision, norm):
+    """The self and n normalisations divide S by another denominator (the
+    pair columns keep N times the shell's mode count): X-ray weights, four
+    species, pair columns."""
+    _adjoint_case(hip_device, precision, "xray", True, 4, norm)
+
+
+def _adjoint_case(hip_device, precision, weight, partials, nspecies, norm):
     import torch
 
     from sqcalc_amd import HipStructureFactor
@@ -43,9 +56,9 @@ def test_adjoint_matches_oracle(hip_device, precision, weight, partials, nspecie
                                               tilt=(1.0, -0.5, 0.3), ntypes=nspecies)
     mapping = MAPPINGS[nspecies]
     qmax, nq = 4.5, 30
-    print("case adjoint", precision, weight, partials, nspecies, flush=True)
+    print("case adjoint", precision, weight, partials, nspecies, norm, flush=True)
     eps = 1e-11 if precision == "double" else 1e-6
-    sf = HipStructureFactor(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax,
+    sf = HipStructureFactor(cell, origin, types, mapping, weight=weight, norm=norm, qmax=qmax,
                             nq=nq, precision=precision, partials=partials, eps=eps)
     rng = np.random.default_rng(4)
     target = torch.tensor(rng.random(nq), device=hip_device)
@@ -59,13 +72,16 @@ def test_adjoint_matches_oracle(hip_device, precision, weight, partials, nspecie
     loss.backward()
     grad = pos.grad.cpu().numpy()
 
-    run = O.Run(cell, origin, types, mapping, weight=weight, norm="mean", qmax=qmax, nq=nq,
+    run = O.Run(cell, origin, types, mapping, weight=weight, norm=norm, qmax=qmax, nq=nq,
                 partials=partials)
     rho = O.direct_rho(frames[0], cell, origin, run.species_of, nspecies, run.md, method=0)
     num, _ = O.shell_sums(rho, run.md, weight, run.amp, run.xcoef)
     Sref = np.where(run.den > 0, num / np.where(run.den > 0, run.den, 1), 0)
+    # 1e-9 / 1e-5 of S's scale (about 1 under mean and self normalisation,
+    # N times larger under n)
+    scale = max(1.0, float(np.abs(Sref).max()))
     np.testing.assert_allclose(S.detach().cpu().numpy(), Sref, rtol=0,
-                               atol=1e-9 if precision == "double" else 1e-5)
+                               atol=(1e-9 if precision == "double" else 1e-5) * scale)
     gS = 2 * (Sref - target.cpu().numpy())
     gP = None
     if partials:
