@@ -1094,6 +1094,23 @@ __global__ void __launch_bounds__(NT) k_spread_z(Geo g, AxisPlan az, const int4 
 // ---------------------------------------------------------------------------
 // TX2: the column chunk at compile time (the tuner picks it among 2, 4, 8
 // and 16), so that a thread's column and row steps are constants.
+
+// A non-temporal load of one complex value, for data read once and larger
+// than the caches.
+template <typename V> __device__ __forceinline__ V nt_load(const V *p) {
+  V v;
+  if constexpr (sizeof(V) == 8) {
+    const unsigned long long b =
+        __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+    __builtin_memcpy(&v, &b, 8);
+  } else {
+    typedef double nd2 __attribute__((ext_vector_type(2)));
+    const nd2 b = __builtin_nontemporal_load(reinterpret_cast<const nd2 *>(p));
+    v.x = b.x;
+    v.y = b.y;
+  }
+  return v;
+}
 template <typename T, int N, int TX2>
 __global__ void __launch_bounds__(NT) k_fft_y(Geo g, AxisPlan ay,
                                               const typename C2<T>::t *__restrict__ W1,
@@ -1329,7 +1346,8 @@ __global__ void __launch_bounds__(NT) k_fft_x_bin(Geo g, AxisPlan ax1, int lb_, 
         [&](int idx) {
           const int r = div_by<N>(idx, inv_n1), x = idx - r * n1;
           const int li = div_by<FC>(r, inv_f), f = r - li * F;
-          return W2[(size_t)(f * g.nlines + l0 + li) * n1 + x];
+          // W2 is read once (non-temporal: 0.210 -> 0.208 ms)
+          return nt_load(W2 + (size_t)(f * g.nlines + l0 + li) * n1 + x);
         },
         [&](int idx, V v) {
           const int r = div_by<N>(idx, inv_n1);
